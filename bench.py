@@ -1,0 +1,236 @@
+"""Benchmark: regularized Stokeslet + stresslet velocity on the band of grid
+nodes around a translating ellipsoid (the extend-to-grid band evaluation).
+
+Workload: ellipsoid semi-axes (1, 0.8, 0.6) at the origin, grid h = 1/64 in
+[-1.5, 1.5]^3, quadrature = grid-line/surface intersections (70 degree
+partition of unity), delta = 4 (1/64)^(2/7) h^(5/7); targets = every grid node
+with |b| <= 0.3.  Densities: f = exact traction of the body translating with
+U = (1,0,0) (SL), g = seeded degree-3 spherical harmonics (DL).  One step =
+one FP64 direct SL_OFF + one DL_OFF evaluation over all band nodes with inputs
+resident in HBM.
+
+Prints one JSON line (metric = pair interactions per second).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [os.path.join(ROOT, "high-order-regularization_amd"), os.path.join(ROOT, "oracle")]
+
+H = 1.0 / 64.0
+BAND = 0.3
+ABC = (1.0, 0.8, 0.6)
+SEED = 20261203
+PEAK_FP64 = 78.6e12          # MI355X FP64 (vector and matrix) FLOP/s
+# Algorithmic operation counts per pair (reference expressions, per-target
+# invariants hoisted, FMA = 2): plain SL 30, regularized off-surface SL 61,
+# plain DL 30, regularized off-surface DL 127.
+OPS = {"sl_plain": 30, "sl_reg": 61, "dl_plain": 30, "dl_reg": 127}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_workload(h=H, band=BAND):
+    from hor import geometry as G
+    body = G.Body(ABC)
+    s = body.surface(h)
+    DEL = G.delta(h)
+    ngrid = int(round(3.0 / h))
+    t0 = time.time()
+    t = body.band_targets(s, (-1.5, -1.5, -1.5), h, ngrid, band)
+    log(f"band targets: {t['n']} nodes in {time.time() - t0:.1f}s; sources {s['n']}; delta {DEL:.5f}")
+    s["f"] = body.traction(s["x"])
+    cg = G.sh_coefficients(SEED + 1)
+    s["g"] = G.sh_density(s["x"], body.ctr, cg)
+    t["f0"] = body.traction(t["x0"])
+    t["g0"] = G.sh_density(t["x0"], body.ctr, cg)
+    return body, s, t, DEL
+
+
+def near_pair_fraction(s, t, DEL, sample=2000, seed=1):
+    """Fraction of (target, source) pairs with r/delta <= 8 (regularized
+    pairs), estimated on a random target sample."""
+    rng = np.random.default_rng(seed)
+    idx = rng.choice(t["n"], min(sample, t["n"]), replace=False)
+    tx = t["x"].reshape(-1, 3)[idx]
+    sx = s["x"].reshape(-1, 3)
+    near = 0
+    for a in range(0, len(tx), 200):
+        d = np.linalg.norm(tx[a:a + 200, None, :] - sx[None, :, :], axis=2)
+        near += int((d / DEL <= 8.0).sum())
+    return near / (len(idx) * s["n"])
+
+
+def cpu_baseline(s, t, DEL, target_seconds=15.0):
+    """The reference's own eval_Stokes_SL_offSurf / eval_Stokes_DL_offSurf
+    (compiled from its sources into oracle/_ref) on a bounded target sample,
+    all host cores; falls back to the oracle restatement when the reference
+    build is absent."""
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    kind = "reference"
+    try:
+        from pyoracle import Ref2
+        impl = Ref2()
+    except (FileNotFoundError, OSError):
+        from pyoracle import Oracle
+        impl, kind = Oracle(), "port"
+    rng = np.random.default_rng(2)
+
+    def sub(n):
+        idx = np.sort(rng.choice(t["n"], n, replace=False))
+        return dict(n=n, x=t["x"].reshape(-1, 3)[idx].ravel(), b=t["b"][idx],
+                    x0=t["x0"].reshape(-1, 3)[idx].ravel(), n0=t["n0"].reshape(-1, 3)[idx].ravel(),
+                    f0=t["f0"].reshape(-1, 3)[idx].ravel(), g0=t["g0"].reshape(-1, 3)[idx].ravel())
+
+    def run(tt):
+        t0 = time.perf_counter()
+        impl.sl_off(DEL, s, tt)
+        impl.dl_off(DEL, s, tt)
+        return time.perf_counter() - t0
+
+    n0 = 1000
+    dt0 = run(sub(n0))
+    n = int(min(t["n"], max(n0, n0 * target_seconds / max(dt0, 1e-3))))
+    tt = sub(n)
+    dt = run(tt)
+    pairs = 2.0 * n * s["n"]
+    return {"value": pairs / dt, "unit": "pair interactions/s", "cores": cores, "kind": kind,
+            "sample": f"{n} of {t['n']} band targets x {s['n']} sources, SL_OFF + DL_OFF, {dt:.1f}s"}
+
+
+def profile_traffic():
+    """HBM bytes per launch of the dominant kernel from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated)."""
+    path = os.path.join(ROOT, "profiles", "direct_band_h64_pmc.json")
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f).get("traffic_bytes_per_launch")
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--h", type=float, default=H)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported (the path shards by "
+              "targets across GPUs by design, but this build measures one GPU)", file=sys.stderr)
+        sys.exit(2)
+    import hor
+
+    body, s, t, DEL = build_workload(h=args.h)
+    ctx = hor.Context(0)
+    t0 = time.time()
+    p_sl = hor.Plan(ctx, hor.SL_OFF, s, t, DEL)
+    p_dl = hor.Plan(ctx, hor.DL_OFF, s, t, DEL)
+    log(f"plans built in {time.time() - t0:.1f}s")
+    ns, nt = s["n"], t["n"]
+    dev = {}
+    for name, arr in (("f", s["f"]), ("g", s["g"]), ("f0", t["f0"]), ("g0", t["g0"])):
+        dev[name] = ctx.alloc(arr.nbytes)
+        ctx.upload(dev[name], np.ascontiguousarray(arr))
+    out_sl, out_dl = ctx.alloc(24 * nt), ctx.alloc(24 * nt)
+
+    def step():
+        p_sl.eval_dev(dev["f"], dev["f0"], out_sl)
+        k_sl = ctx.last_timing()[1]
+        p_dl.eval_dev(dev["g"], dev["g0"], out_dl)
+        k_dl = ctx.last_timing()[1]
+        return k_sl, k_dl
+
+    for _ in range(args.warmup):
+        step()
+    ksl, kdl = [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        a, b = step()
+        ksl.append(a)
+        kdl.append(b)
+    wall = time.perf_counter() - t0
+    ms_per_step = 1e3 * wall / args.steps
+    pairs_per_step = 2.0 * ns * nt
+    value = pairs_per_step / (wall / args.steps)
+
+    # mixed-precision variant (FP32 far pairs, FP64 accumulation): time and error
+    m_sl = hor.Plan(ctx, hor.SL_OFF, s, t, DEL, precision=hor.MIXED)
+    m_dl = hor.Plan(ctx, hor.DL_OFF, s, t, DEL, precision=hor.MIXED)
+    ref_sl = ctx.download(out_sl, np.zeros(3 * nt))
+    ref_dl = ctx.download(out_dl, np.zeros(3 * nt))
+    m_sl.eval_dev(dev["f"], dev["f0"], out_sl)
+    m_dl.eval_dev(dev["g"], dev["g0"], out_dl)
+    tm = time.perf_counter()
+    for _ in range(args.steps):
+        m_sl.eval_dev(dev["f"], dev["f0"], out_sl)
+        m_dl.eval_dev(dev["g"], dev["g0"], out_dl)
+    mixed_ms = 1e3 * (time.perf_counter() - tm) / args.steps
+    mix_sl = ctx.download(out_sl, np.zeros(3 * nt))
+    mix_dl = ctx.download(out_dl, np.zeros(3 * nt))
+    err_sl = float(np.linalg.norm(mix_sl - ref_sl) / np.linalg.norm(ref_sl))
+    err_dl = float(np.linalg.norm(mix_dl - ref_dl) / np.linalg.norm(ref_dl))
+
+    # error against the exact translating-body velocity: u = U = (1,0,0) inside
+    inside = t["b"] < 0
+    u_sl = ref_sl.reshape(-1, 3)
+    err_inside = float(np.abs(u_sl[inside] - np.array([1.0, 0.0, 0.0])).max()) if inside.any() else None
+
+    # roofline of the dominant kernel (DL_OFF direct sum): algorithmic FP64
+    # flops per launch / average launch time (HIP events on the plan's stream)
+    fnear = near_pair_fraction(s, t, DEL)
+    dl_flops = ns * nt * (fnear * OPS["dl_reg"] + (1 - fnear) * OPS["dl_plain"])
+    sl_flops = ns * nt * (fnear * OPS["sl_reg"] + (1 - fnear) * OPS["sl_plain"])
+    kdl_ms = float(np.mean(kdl))
+    ksl_ms = float(np.mean(ksl))
+    achieved = dl_flops / (kdl_ms * 1e-3) / 1e12
+    roof = {"bound": "mfma", "compute": "fp64 vector ALU (peak equals the FP64 matrix peak)",
+            "kernel": "k_direct<DL_OFF,FP64>", "achieved": achieved, "peak": PEAK_FP64 / 1e12,
+            "unit": "TFLOP/s", "frac": achieved * 1e12 / PEAK_FP64, "traffic": profile_traffic(),
+            "avg_kernel_ms": kdl_ms, "sl_kernel_ms": ksl_ms,
+            "sl_achieved_tflops": sl_flops / (ksl_ms * 1e-3) / 1e12,
+            "near_pair_fraction": fnear}
+
+    cpu = None if args.no_cpu else cpu_baseline(s, t, DEL)
+    line = {
+        "metric": "pair interactions/s",
+        "value": value,
+        "unit": "pair interactions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "extend-to-grid band: ellipsoid (1,0.8,0.6), h=1/64, box [-1.5,1.5]^3, "
+                               "|b|<=0.3, FP64 direct SL_OFF + DL_OFF",
+                   "sources": ns, "targets": nt, "delta": DEL, "pairs_per_step": pairs_per_step},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "mixed": {"ms_per_step": mixed_ms, "value": pairs_per_step / (mixed_ms * 1e-3),
+                  "rel_err_vs_fp64_sl": err_sl, "rel_err_vs_fp64_dl": err_dl},
+        "accuracy": {"sl_max_err_inside_vs_exact_U": err_inside},
+    }
+    for p in dev.values():
+        ctx.free(p)
+    ctx.free(out_sl)
+    ctx.free(out_dl)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,148 @@
+// Device-side building blocks shared by the sum kernels (HIP, gfx950).
+//
+// Pair arithmetic follows the reference's per-pair expressions
+// (highreg_stokes_2surf/utilities.cpp:179-323, stokeslets.cpp:106-130,
+// stresslets.cpp:117-168, 244-269).  The per-target regularization
+// polynomial coefficients are precomputed on the host (TgtCoef below).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "hor_internal.h"
+
+#define HOR_HIP_TRY(expr)                                                                  \
+  do {                                                                                     \
+    hipError_t e_ = (expr);                                                                \
+    if (e_ != hipSuccess)                                                                  \
+      return hor::set_error(HOR_E_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                            __FILE__, __LINE__);                                           \
+  } while (0)
+
+#define HOR_TRY(expr)        \
+  do {                       \
+    int r_ = (expr);         \
+    if (r_ != HOR_OK) return r_; \
+  } while (0)
+
+struct hor_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  double last_total_ms = 0, last_kernel_ms = 0;
+};
+
+namespace hor {
+
+constexpr int kBlock = 256;     // threads per workgroup (4 waves of 64)
+constexpr int kWave = 64;
+constexpr double kPi = 3.14159265358979323846;
+
+// Device buffer owned by a plan; freed on destruction.
+template <class T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  int alloc(size_t count) {
+    release();
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc(&p, count * sizeof(T));
+    if (e != hipSuccess) {
+      p = nullptr;
+      return set_error(HOR_E_NOMEM, "hipMalloc(%zu bytes) failed: %s", count * sizeof(T),
+                       hipGetErrorString(e));
+    }
+    n = count;
+    return HOR_OK;
+  }
+  int upload(const T* src, size_t count, hipStream_t s) {
+    HOR_TRY(alloc(count));
+    if (count) {
+      hipError_t e = hipMemcpyAsync(p, src, count * sizeof(T), hipMemcpyHostToDevice, s);
+      if (e != hipSuccess) return set_error(HOR_E_HIP, "upload: %s", hipGetErrorString(e));
+    }
+    return HOR_OK;
+  }
+  int upload(const std::vector<T>& v, hipStream_t s) { return upload(v.data(), v.size(), s); }
+};
+
+// ---------------------------------------------------------------------------
+// Per-target regularization data, structure-of-arrays with stride ntp.
+//   SL: s1 = (erf(rho) + P1(rho^2) rho e^{-rho^2}) / r,
+//       s2 = (erf(rho) + P2(rho^2) rho e^{-rho^2}) / r^3
+//       P1 = q0 + q1 rho^2 + q2 rho^4, P2 = q3 + q4 rho^2 + q5 rho^4 + q6 rho^6
+//       (utilities.cpp:199-205; on the surface c = (22/5, 8/5, 2/15)/sqrt(pi),
+//       which reproduces Stokes_SL_7ord_onSurf, utilities.cpp:236-252)
+//   DL: s2 as P2 above, s3 = (erf + P3 rho e^{-rho^2}) / r^5,
+//       P3 = q7 + q8 rho^2 + q9 rho^4 + q10 rho^6 + q11 rho^8 (utilities.cpp:280-282)
+//       on the surface P3 = 2/sqrt(pi) (-1 - 2/3 rho^2 + 52/45 rho^4 - 8/45 rho^6)
+//       (utilities.cpp:310-315).
+constexpr int kNumCoef = 12;
+
+// Fills the 12 polynomial coefficients from c1, c2, c3.
+inline void poly_coefs(const double c[3], double q[kNumCoef]) {
+  const double rp = 1.7724538509055160273;  // sqrt(pi)
+  double c1 = c[0], c2 = c[1], c3 = c[2];
+  q[0] = c1;
+  q[1] = -2.0 * (c2 + c3);
+  q[2] = 4.0 * c3;
+  q[3] = -2.0 / rp;
+  q[4] = 2.0 * (c1 + 2.0 * (c2 + c3));
+  q[5] = -4.0 * (c2 + 5.0 * c3);
+  q[6] = 8.0 * c3;
+  q[7] = -2.0 / rp;
+  q[8] = -2.0 / rp * (2.0 / 3.0);
+  q[9] = 4.0 / 3.0 * (c1 + 4.0 * c2 + 12.0 * c3);
+  q[10] = -8.0 / 3.0 * (c2 + 9.0 * c3);
+  q[11] = 16.0 / 3.0 * c3;
+}
+
+// On-surface stresslet polynomial (7th order), already multiplied by 2/sqrt(pi).
+inline void poly_coefs_dl_on(double q[kNumCoef]) {
+  const double k = 2.0 / 1.7724538509055160273;
+  for (int i = 0; i < kNumCoef; i++) q[i] = 0.0;
+  q[7] = -1.0 * k;
+  q[8] = -2.0 / 3.0 * k;
+  q[9] = 52.0 / 45.0 * k;
+  q[10] = -8.0 / 45.0 * k;
+  q[11] = 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// Device math.
+
+// 1/sqrt(x) in FP64: hardware estimate refined by two Newton-Raphson steps
+// (each y <- y + y/2 (1 - x y^2)); error below 2 ulp for normal x.
+__device__ __forceinline__ double rsqrt64(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  double e = __builtin_fma(-x * y, y, 1.0);
+  y = __builtin_fma(0.5 * y, e, y);
+  e = __builtin_fma(-x * y, y, 1.0);
+  y = __builtin_fma(0.5 * y, e, y);
+  return y;
+}
+
+__device__ __forceinline__ float rsqrt32(float x) { return __builtin_amdgcn_rsqf(x); }
+
+// Source record staged in LDS: position, weight, normal, density.
+struct __align__(16) SrcRec {
+  double x, y, z, a;
+  double nx, ny, nz, pad0;
+  double d0, d1, d2, pad1;
+};
+
+struct __align__(16) SrcRec32 {
+  float x, y, z, a;
+  float nx, ny, nz, pad0;
+  float d0, d1, d2, pad1;
+};
+
+}  // namespace hor

@@ -1,0 +1,44 @@
+// Kernel argument blocks and launchers of the sum kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "hor_internal.h"
+
+namespace hor {
+
+// Direct sums / treecode leaf sums.  All device pointers.  Sources are in the
+// plan's sorted order as structure-of-arrays with stride nsp; targets in the
+// plan's sorted order with stride ntp (a multiple of 256).
+struct DirectArgs {
+  int ns, nsp, nt, ntp;
+  int ntile, tiles_per_chunk;      // 256-source LDS tiles, tiles per grid.y chunk
+  double DEL, far_d2;              // delta, (8 delta (1+1e-10))^2
+  const double* sx;                // 3*nsp positions
+  const double* sn;                // 3*nsp normals
+  const double* sa;                // nsp weights
+  const double* sd;                // 3*nsp density (f or g)
+  const double* sub_box;           // 6 per 64-source subtile (lo xyz, hi xyz)
+  const double* tx;                // 3*ntp positions
+  const double* tcoef;             // 12*ntp regularization polynomial coefficients
+  const double* tsub;              // SL: ntp f0.n0; DL: 3*ntp g0
+  const double* tx0;               // DL_OFF: 3*ntp nearest point
+  const double* tn0;               // DL_OFF: 3*ntp nearest normal
+  const double* tb;                // DL_OFF: ntp signed distance
+  const int* tnear;                // ntp: 1 = regularized target (|b|/delta <= 8 or on surface)
+  const double* wave_box;          // 8 per 64-target group: lo xyz, hi xyz, all-far flag, pad
+  double* partial;                 // nchunk*3*ntp partial sums
+};
+
+int launch_direct(const DirectArgs& A, int kind, bool mixed, int nchunk, hipStream_t s);
+int launch_finalize(const double* partial, int nchunk, int nt, int ntp, double scale,
+                    const double* xi, const double* g0, const int* tperm, double* out,
+                    const double* base, double coef, int accumulate, hipStream_t s);
+int launch_gather3(const double* dens, const int* perm, int n, int np, double* sd, hipStream_t s);
+int launch_target_sub(int mode, int nt, int ntp, const int* tperm, const int* tnear,
+                      const double* d0, const double* n0_orig, const double* dens,
+                      const double* sn_orig, double* out, hipStream_t s);
+int launch_target_f0n0_all(int nt, int ntp, const int* tperm, const double* d0,
+                           const double* n0_orig, double* out, hipStream_t s);
+
+}  // namespace hor

@@ -1,0 +1,201 @@
+// Direct regularized Stokeslet / stresslet sums on gfx950.
+//
+// One thread owns one target (targets sorted far-first, then by Morton key,
+// so a wave's 64 targets are spatially compact and share one regularization
+// class).  A workgroup of 256 targets streams the sources (Morton sorted)
+// through 256-entry LDS tiles; each wave walks a tile in 64-source subtiles
+// and, per subtile, takes one of two wave-uniform paths:
+//   * fast path: the subtile's bounding box lies beyond 8*delta (plus a
+//     rounding margin) of the wave's target box, or every target of the wave
+//     is beyond 8*delta of the surface: every pair uses the plain kernel
+//     (utilities.cpp:186-190, 265-269), no erf/exp, no branch;
+//   * mixed path: per pair, self limit / plain / regularized exactly as
+//     Stokes_SL_7ord / Stokes_DL_7ord (utilities.cpp:179-323).
+// Sources are split into chunks across grid.y when there are too few target
+// workgroups to fill the 256 CUs; each chunk writes a partial sum and a
+// finalize kernel adds the chunks in a fixed order (deterministic).
+#include "pairs.h"
+
+namespace hor {
+
+template <int KIND, bool MIXED>
+__global__ __launch_bounds__(kBlock) void k_direct(DirectArgs A) {
+  __shared__ SrcRec tile[kBlock];
+  __shared__ SrcRec32 tile32[MIXED ? kBlock : 1];
+  const int tid = threadIdx.x;
+  const int i = blockIdx.x * kBlock + tid;     // sorted target (ntp is a multiple of kBlock)
+  const int wave = i / kWave;
+  Tgt t;
+  load_target<KIND>(A, i, t);
+  const double* wb = A.wave_box + 8 * wave;
+  const bool all_far = wb[6] != 0.0;
+  float pxf = (float)t.px, pyf = (float)t.py, pzf = (float)t.pz;
+  double u[3] = {0.0, 0.0, 0.0};
+  const int tile0 = blockIdx.y * A.tiles_per_chunk;
+  const int tile1 = min(A.ntile, tile0 + A.tiles_per_chunk);
+  for (int tl = tile0; tl < tile1; tl++) {
+    const int base = tl * kBlock;
+    const int len = min(kBlock, A.ns - base);
+    __syncthreads();
+    if (tid < len) {
+      int j = base + tid;
+      SrcRec r;
+      r.x = A.sx[j]; r.y = A.sx[A.nsp + j]; r.z = A.sx[2 * A.nsp + j]; r.a = A.sa[j];
+      r.nx = A.sn[j]; r.ny = A.sn[A.nsp + j]; r.nz = A.sn[2 * A.nsp + j]; r.pad0 = 0;
+      r.d0 = A.sd[j]; r.d1 = A.sd[A.nsp + j]; r.d2 = A.sd[2 * A.nsp + j]; r.pad1 = 0;
+      tile[tid] = r;
+      if (MIXED) {
+        SrcRec32 q;
+        q.x = (float)r.x; q.y = (float)r.y; q.z = (float)r.z; q.a = (float)r.a;
+        q.nx = (float)r.nx; q.ny = (float)r.ny; q.nz = (float)r.nz; q.pad0 = 0;
+        q.d0 = (float)r.d0; q.d1 = (float)r.d1; q.d2 = (float)r.d2; q.pad1 = 0;
+        tile32[tid] = q;
+      }
+    }
+    __syncthreads();
+    tile_sum<KIND, MIXED>(A, tile, tile32, len, base / kWave, wb, all_far, t, pxf, pyf, pzf, u);
+  }
+  double* P = A.partial + (size_t)blockIdx.y * 3 * A.ntp;
+  P[i] = u[0];
+  P[A.ntp + i] = u[1];
+  P[2 * A.ntp + i] = u[2];
+}
+
+// Adds the chunk partials in chunk order, scales, adds the jump term and
+// writes v = scale*sum + xi*g0 in the caller's order (xyz interleaved):
+//   out = coef*v                 (accumulate = 0, base = NULL)
+//   out = base + coef*v          (accumulate = 0; the solver's u - 2/3 DL)
+//   out = out + coef*v           (accumulate = 1)
+__global__ void k_direct_finalize(const double* partial, int nchunk, int nt, int ntp, double scale,
+                                  const double* xi, const double* g0, const int* tperm, double* out,
+                                  const double* base, double coef, int accumulate) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nt) return;
+  int o = tperm[i];
+  for (int k = 0; k < 3; k++) {
+    double s = 0.0;
+    for (int c = 0; c < nchunk; c++) s += partial[((size_t)c * 3 + k) * ntp + i];
+    double v = s * scale;
+    if (xi) v += xi[i] * g0[(size_t)k * ntp + i];
+    double w = coef * v;
+    if (accumulate) out[3 * o + k] += w;
+    else if (base) out[3 * o + k] = base[3 * o + k] + w;
+    else out[3 * o + k] = w;
+  }
+}
+
+// Density gather into sorted SoA: sd[k*nsp + j] = dens[3*perm[j] + k].
+__global__ void k_gather3(const double* dens, const int* perm, int n, int np, double* sd) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  int o = perm[j];
+  sd[j] = dens[3 * o];
+  sd[np + j] = dens[3 * o + 1];
+  sd[2 * np + j] = dens[3 * o + 2];
+}
+
+// Per-target subtraction data (sorted order):
+//   mode 0 (SL_OFF): sub = near ? d0.n0 : 0       (stokeslets.cpp:72)
+//   mode 1 (SL_ON):  sub = f_o . n_o of source o  (stokeslets.cpp:167)
+//   mode 2 (DL_OFF): g0 = near ? d0 : 0            (leaf / direct sums)
+//   mode 3 (DL_ON):  g0 = g_o of source o          (stresslets.cpp:212-214)
+//   mode 4 (DL_OFF): g0 = d0 for every target      (treecode proxies and jump)
+__global__ void k_target_sub(int mode, int nt, int ntp, const int* tperm, const int* tnear,
+                             const double* d0, const double* n0_orig, const double* dens,
+                             const double* sn_orig, double* out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ntp) return;
+  if (i >= nt) {
+    out[i] = 0;
+    if (mode >= 2) { out[ntp + i] = 0; out[2 * ntp + i] = 0; }
+    return;
+  }
+  int o = tperm[i];
+  bool nearp = tnear[i] != 0;
+  if (mode == 0) {
+    out[i] = nearp ? d0[3 * o] * n0_orig[3 * o] + d0[3 * o + 1] * n0_orig[3 * o + 1] +
+                         d0[3 * o + 2] * n0_orig[3 * o + 2]
+                   : 0.0;
+  } else if (mode == 1) {
+    out[i] = dens[3 * o] * sn_orig[3 * o] + dens[3 * o + 1] * sn_orig[3 * o + 1] +
+             dens[3 * o + 2] * sn_orig[3 * o + 2];
+  } else if (mode == 2) {
+    for (int k = 0; k < 3; k++) out[(size_t)k * ntp + i] = nearp ? d0[3 * o + k] : 0.0;
+  } else if (mode == 3) {
+    for (int k = 0; k < 3; k++) out[(size_t)k * ntp + i] = dens[3 * o + k];
+  } else {
+    for (int k = 0; k < 3; k++) out[(size_t)k * ntp + i] = d0[3 * o + k];
+  }
+}
+
+// SL_OFF proxy subtraction uses f0.n0 for every target (KITC.cpp:1568).
+__global__ void k_target_f0n0_all(int nt, int ntp, const int* tperm, const double* d0,
+                                  const double* n0_orig, double* out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ntp) return;
+  if (i >= nt) { out[i] = 0; return; }
+  int o = tperm[i];
+  out[i] = d0[3 * o] * n0_orig[3 * o] + d0[3 * o + 1] * n0_orig[3 * o + 1] + d0[3 * o + 2] * n0_orig[3 * o + 2];
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers.
+
+int launch_direct(const DirectArgs& A, int kind, bool mixed, int nchunk, hipStream_t s) {
+  dim3 grid(A.ntp / kBlock, nchunk), block(kBlock);
+#define HOR_LAUNCH(K, M) hipLaunchKernelGGL((k_direct<K, M>), grid, block, 0, s, A)
+  if (!mixed) {
+    switch (kind) {
+      case HOR_SL_ON: HOR_LAUNCH(HOR_SL_ON, false); break;
+      case HOR_SL_OFF: HOR_LAUNCH(HOR_SL_OFF, false); break;
+      case HOR_DL_ON: HOR_LAUNCH(HOR_DL_ON, false); break;
+      default: HOR_LAUNCH(HOR_DL_OFF, false); break;
+    }
+  } else {
+    switch (kind) {
+      case HOR_SL_ON: HOR_LAUNCH(HOR_SL_ON, true); break;
+      case HOR_SL_OFF: HOR_LAUNCH(HOR_SL_OFF, true); break;
+      case HOR_DL_ON: HOR_LAUNCH(HOR_DL_ON, true); break;
+      default: HOR_LAUNCH(HOR_DL_OFF, true); break;
+    }
+  }
+#undef HOR_LAUNCH
+  HOR_HIP_TRY(hipGetLastError());
+  return HOR_OK;
+}
+
+int launch_finalize(const double* partial, int nchunk, int nt, int ntp, double scale, const double* xi,
+                    const double* g0, const int* tperm, double* out, const double* base,
+                    double coef, int accumulate, hipStream_t s) {
+  if (nt == 0) return HOR_OK;
+  hipLaunchKernelGGL(k_direct_finalize, dim3((nt + 255) / 256), dim3(256), 0, s, partial, nchunk, nt,
+                     ntp, scale, xi, g0, tperm, out, base, coef, accumulate);
+  HOR_HIP_TRY(hipGetLastError());
+  return HOR_OK;
+}
+
+int launch_gather3(const double* dens, const int* perm, int n, int np, double* sd, hipStream_t s) {
+  if (n == 0) return HOR_OK;
+  hipLaunchKernelGGL(k_gather3, dim3((n + 255) / 256), dim3(256), 0, s, dens, perm, n, np, sd);
+  HOR_HIP_TRY(hipGetLastError());
+  return HOR_OK;
+}
+
+int launch_target_sub(int mode, int nt, int ntp, const int* tperm, const int* tnear, const double* d0,
+                      const double* n0_orig, const double* dens, const double* sn_orig, double* out,
+                      hipStream_t s) {
+  hipLaunchKernelGGL(k_target_sub, dim3((ntp + 255) / 256), dim3(256), 0, s, mode, nt, ntp, tperm,
+                     tnear, d0, n0_orig, dens, sn_orig, out);
+  HOR_HIP_TRY(hipGetLastError());
+  return HOR_OK;
+}
+
+int launch_target_f0n0_all(int nt, int ntp, const int* tperm, const double* d0, const double* n0_orig,
+                           double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_target_f0n0_all, dim3((ntp + 255) / 256), dim3(256), 0, s, nt, ntp, tperm, d0,
+                     n0_orig, out);
+  HOR_HIP_TRY(hipGetLastError());
+  return HOR_OK;
+}
+
+}  // namespace hor

@@ -1,0 +1,720 @@
+// Host geometry: the input producers of the hot path (run once per geometry).
+//
+// Quadrature by grid-line/surface intersection with partition-of-unity
+// weights, nearest-point data of target points, surface forcing, band targets
+// and the translating-body densities / exact velocity.  Each function follows
+// the reference file:line given beside it (paths relative to the reference
+// root) and keeps its arithmetic order where the result feeds a parity check.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "hor_internal.h"
+
+namespace hor {
+
+static const double kPi = 3.14159265358979323846;
+
+// ---------------------------------------------------------------------------
+// Level set of an ellipsoid body, x_body = R (x - ctr)
+// (highreg_stokes_2surf/utilities.cpp:44-158).
+
+static inline void to_body(const hor_body& B, const double x[3], bool shift, double u[3]) {
+  double xx[3];
+  for (int k = 0; k < 3; k++) xx[k] = shift ? x[k] - B.ctr[k] : x[k];
+  if (B.rotated) {
+    for (int i = 0; i < 3; i++) {
+      double s = 0.0;
+      for (int j = 0; j < 3; j++) s += B.R[3 * i + j] * xx[j];
+      u[i] = s;
+    }
+  } else {
+    u[0] = xx[0]; u[1] = xx[1]; u[2] = xx[2];
+  }
+}
+
+// phi (utilities.cpp:58-86); shift=false evaluates in the body-centred frame.
+static inline double phi(const hor_body& B, const double x[3], bool shift) {
+  double u[3];
+  to_body(B, x, shift, u);
+  double a2[3] = {B.abc[0] * B.abc[0], B.abc[1] * B.abc[1], B.abc[2] * B.abc[2]};
+  return u[0] * u[0] / a2[0] + u[1] * u[1] / a2[1] + u[2] * u[2] / a2[2] - 1.0;
+}
+
+// Dphi (utilities.cpp:91-121).
+static inline double dphi(const hor_body& B, int i, const double x[3], bool shift) {
+  double a2[3] = {B.abc[0] * B.abc[0], B.abc[1] * B.abc[1], B.abc[2] * B.abc[2]};
+  if (B.rotated) {
+    double u[3];
+    to_body(B, x, shift, u);
+    double g[3] = {2.0 * u[0] / a2[0], 2.0 * u[1] / a2[1], 2.0 * u[2] / a2[2]};
+    return g[0] * B.R[0 * 3 + i] + g[1] * B.R[1 * 3 + i] + g[2] * B.R[2 * 3 + i];
+  }
+  double xi = shift ? x[i] - B.ctr[i] : x[i];
+  return 2.0 * xi / a2[i];
+}
+
+// D2phi (utilities.cpp:126-158): constant Hessian.
+static void d2phi(const hor_body& B, double H[9]) {
+  double a2[3] = {B.abc[0] * B.abc[0], B.abc[1] * B.abc[1], B.abc[2] * B.abc[2]};
+  if (B.rotated) {
+    double A[3] = {2.0 / a2[0], 2.0 / a2[1], 2.0 / a2[2]};
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++)
+        H[3 * i + j] = A[0] * B.R[i] * B.R[j] + A[1] * B.R[3 + i] * B.R[3 + j] +
+                       A[2] * B.R[6 + i] * B.R[6 + j];
+    H[3] = H[1]; H[6] = H[2]; H[7] = H[5];
+  } else {
+    for (int k = 0; k < 9; k++) H[k] = 0.0;
+    H[0] = 2.0 / a2[0]; H[4] = 2.0 / a2[1]; H[8] = 2.0 / a2[2];
+  }
+}
+
+static inline int sign_phi(const hor_body& B, const double x[3]) {
+  return phi(B, x, false) < 0.0 ? -1 : 1;  // surface.cpp:76-81
+}
+
+static const double kTol = 1e-14;  // highreg_stokes_2surf/utilities.h:14
+
+// bisection (surface.cpp:139-174), on the segment PT + t*H e_i, t in [a,b].
+static double bisection(const hor_body& B, const double PT[3], double H, int i, double a, double b) {
+  double xl[3] = {PT[0], PT[1], PT[2]}, xr[3] = {PT[0], PT[1], PT[2]}, xm[3] = {PT[0], PT[1], PT[2]};
+  xl[i] += a * H;
+  xr[i] += b * H;
+  double p = 0.0;
+  if (phi(B, xl, false) * phi(B, xr, false) > 0.0) return p;
+  if (std::fabs(phi(B, xl, false)) <= kTol) return a;
+  if (std::fabs(phi(B, xr, false)) <= kTol) return b;
+  p = (a + b) / 2;
+  xm[i] = PT[i] + p * H;
+  double pm = phi(B, xm, false);
+  int guard = 0;
+  while (std::fabs(pm) > kTol && guard++ < 200) {
+    if (phi(B, xl, false) * pm < 0.0) b = p;
+    else { a = p; std::memcpy(xl, xm, sizeof(xl)); }
+    p = (a + b) / 2;
+    xm[i] = PT[i] + p * H;
+    pm = phi(B, xm, false);
+  }
+  return p;
+}
+
+// Newton (surface.cpp:95-135) with the bisection fallback.
+static double newton(const hor_body& B, const double PT[3], double H, int i, double a, double b) {
+  const int n_max = 100;
+  double xl[3] = {PT[0], PT[1], PT[2]}, xr[3] = {PT[0], PT[1], PT[2]}, xn[3] = {PT[0], PT[1], PT[2]};
+  xl[i] += a * H;
+  xr[i] += b * H;
+  double p = 0.0;
+  if (phi(B, xl, false) * phi(B, xr, false) > 0.0) return p;
+  if (std::fabs(phi(B, xl, false)) <= kTol) return a;
+  if (std::fabs(phi(B, xr, false)) <= kTol) return b;
+  p = (a + b) / 2;
+  xn[i] = PT[i] + p * H;
+  double pn = phi(B, xn, false);
+  double dn = dphi(B, i, xn, false) * H;
+  int n = 0;
+  while ((std::fabs(pn) > kTol) && (std::fabs(dn) > kTol) && (n < n_max)) {
+    p -= pn / dn;
+    xn[i] = PT[i] + p * H;
+    n = n + 1;
+    pn = phi(B, xn, false);
+    dn = dphi(B, i, xn, false) * H;
+  }
+  if ((n >= n_max) || (std::fabs(dn) <= kTol)) p = bisection(B, PT, H, i, a, b);
+  return p;
+}
+
+// bump and Part_Unity (surface.cpp:181-207).
+static inline double bump(double r) {
+  if (std::fabs(r) < 1.0) {
+    double r2 = r * r;
+    return std::exp(2.0 * r2 / (r2 - 1.0));
+  }
+  return 0.0;
+}
+
+static inline double part_unity(int i, const double n[3], double theta) {
+  double v0 = bump(std::acos(std::fabs(n[0])) / theta);
+  double v1 = bump(std::acos(std::fabs(n[1])) / theta);
+  double v2 = bump(std::acos(std::fabs(n[2])) / theta);
+  double v[3] = {v0, v1, v2};
+  double s = v0 + v1 + v2;
+  return v[i] / s;
+}
+
+// Generate_Surface (surface.cpp:13-71) in the body-centred frame, then
+// translated by the centre (highreg_stokes_2surf/main.cpp:89-98).
+int surface_generate(const hor_body& B, double h, double pou_deg, std::vector<double>& X,
+                     std::vector<double>& Nr, std::vector<double>& Ar, std::vector<int>& Pl) {
+  double rmax = std::max(B.abc[0], std::max(B.abc[1], B.abc[2]));
+  int N = (int)std::ceil(rmax / h) + 1;
+  double theta = pou_deg * kPi / 180.0;
+  double ct = std::cos(theta);
+  double A[3] = {0, 0, 0}, Bp[3] = {0, 0, 0};
+  for (int i = 0; i < 3; i++) {
+    for (int l1 = -N; l1 <= N; l1++) {
+      for (int l2 = -N; l2 <= N; l2++) {
+        if (i == 0) { A[1] = l1 * h; A[2] = l2 * h; Bp[1] = l1 * h; Bp[2] = l2 * h; }
+        if (i == 1) { A[0] = l1 * h; A[2] = l2 * h; Bp[0] = l1 * h; Bp[2] = l2 * h; }
+        if (i == 2) { A[0] = l1 * h; A[1] = l2 * h; Bp[0] = l1 * h; Bp[1] = l2 * h; }
+        for (int j = -N; j <= N; j++) {
+          A[i] = j * h;
+          Bp[i] = A[i] + h;
+          if (sign_phi(B, A) * sign_phi(B, Bp) < 0) {
+            double tau = newton(B, A, h, i, 0, 1);
+            double x[3] = {A[0], A[1], A[2]};
+            x[i] += tau * h;
+            double n[3];
+            for (int k = 0; k < 3; k++) n[k] = dphi(B, k, x, false);
+            double nn = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+            for (int k = 0; k < 3; k++) n[k] /= nn;
+            if (std::fabs(n[i]) >= ct) {
+              double area = h * h * part_unity(i, n, theta) / std::fabs(n[i]);
+              for (int k = 0; k < 3; k++) {
+                X.push_back(x[k] + B.ctr[k]);
+                Nr.push_back(n[k]);
+              }
+              Ar.push_back(area);
+              Pl.push_back(i);
+            }
+          }
+        }
+      }
+    }
+  }
+  return (int)Ar.size();
+}
+
+// ---------------------------------------------------------------------------
+// 4x4 LU with partial pivoting (utilities.cpp:361-431).
+
+static int lu4(double A[4][4], int p[4]) {
+  const int n = 4;
+  for (int j = 0; j < n; j++) p[j] = j;
+  for (int j = 0; j < n; j++) {
+    int k = j;
+    double m = A[p[j]][j];
+    for (int i = j + 1; i < n; i++)
+      if (std::fabs(A[p[i]][j]) > std::fabs(m)) { m = A[p[i]][j]; k = i; }
+    if (k != j) std::swap(p[j], p[k]);
+    double ajj = A[p[j]][j];
+    if (std::fabs(ajj) < 1.0e-15) return -1;
+    for (int i = j + 1; i < n; i++) {
+      double lij = A[p[i]][j] / ajj;
+      A[p[i]][j] = lij;
+      for (int kk = j + 1; kk < n; kk++) A[p[i]][kk] -= lij * A[p[j]][kk];
+    }
+  }
+  return 0;
+}
+
+static void lu4_solve(double A[4][4], const int p[4], const double b[4], double x[4]) {
+  const int n = 4;
+  x[0] = b[p[0]];
+  for (int i = 1; i < n; i++) {
+    x[i] = b[p[i]];
+    double s = 0.0;
+    for (int j = 0; j < i; j++) s += A[p[i]][j] * x[j];
+    x[i] -= s;
+  }
+  x[n - 1] = x[n - 1] / A[p[n - 1]][n - 1];
+  for (int i = n - 2; i >= 0; i--) {
+    double s = 0.0;
+    for (int j = n - 1; j > i; j--) s += A[p[i]][j] * x[j];
+    x[i] = (x[i] - s) / A[p[i]][i];
+  }
+}
+
+// find_Nearest_Start (surface.cpp:277-365): Newton on (lambda, x) for the
+// closest point of the level set to pt, started from xs.
+static int nearest_point(const hor_body& B, const double pt[3], const double xs[3], double x0[3],
+                         double n0[3]) {
+  const int n_max = 100;
+  double x[3] = {xs[0], xs[1], xs[2]};
+  double lam = 0.0;
+  int n = 0;
+  double F[4];
+  F[0] = phi(B, x, true);
+  F[1] = x[0] - pt[0];
+  F[2] = x[1] - pt[1];
+  F[3] = x[2] - pt[2];
+  double Fn2 = F[0] * F[0] + F[1] * F[1] + F[2] * F[2] + F[3] * F[3];
+  double H[9];
+  d2phi(B, H);
+  while ((Fn2 > kTol) && (n <= n_max)) {
+    double g[3] = {dphi(B, 0, x, true), dphi(B, 1, x, true), dphi(B, 2, x, true)};
+    F[0] = phi(B, x, true);
+    F[1] = x[0] - pt[0] + lam * g[0];
+    F[2] = x[1] - pt[1] + lam * g[1];
+    F[3] = x[2] - pt[2] + lam * g[2];
+    Fn2 = F[0] * F[0] + F[1] * F[1] + F[2] * F[2] + F[3] * F[3];
+    double J[4][4] = {{0.0, g[0], g[1], g[2]},
+                      {g[0], 1.0 + lam * H[0], lam * H[1], lam * H[2]},
+                      {g[1], lam * H[3], 1.0 + lam * H[4], lam * H[5]},
+                      {g[2], lam * H[6], lam * H[7], 1.0 + lam * H[8]}};
+    int p[4];
+    if (lu4(J, p) == -1) return -1;
+    double s[4];
+    lu4_solve(J, p, F, s);
+    lam = lam - s[0];
+    for (int i = 0; i < 3; i++) x[i] = x[i] - s[i + 1];
+    n = n + 1;
+  }
+  if (n > n_max) return -1;
+  double g[3] = {dphi(B, 0, x, true), dphi(B, 1, x, true), dphi(B, 2, x, true)};
+  double nn = std::sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+  for (int k = 0; k < 3; k++) {
+    x0[k] = x[k];
+    n0[k] = g[k] / nn;
+  }
+  return 0;
+}
+
+// select_Monge_Patch (surface.cpp:395-431).
+static void monge_patch(const hor_body& B, const double x[3], int& i1, int& i2, int& i3) {
+  double nx = std::fabs(dphi(B, 0, x, true));
+  double ny = std::fabs(dphi(B, 1, x, true));
+  double nz = std::fabs(dphi(B, 2, x, true));
+  if (nx >= ny) {
+    if (nx >= nz) { i1 = 1; i2 = 2; i3 = 0; }
+    else { i1 = 0; i2 = 1; i3 = 2; }
+  } else {
+    if (ny >= nz) { i1 = 0; i2 = 2; i3 = 1; }
+    else { i1 = 0; i2 = 1; i3 = 2; }
+  }
+}
+
+// Mean curvature (surface.cpp:214-272): the reference's spheroid formula when
+// the body is a spheroid about its first axis (b == c), else the general
+// level-set formula that follows it in the same function.
+static double mean_curvature(const hor_body& B, const double x[3]) {
+  if (B.abc[1] == B.abc[2]) {
+    double a = B.abc[1], c = B.abc[0];
+    double a2 = a * a;
+    double u[3];
+    to_body(B, x, true, u);
+    double cos2 = (u[1] * u[1] + u[2] * u[2]) / a2;
+    double t = a2 + (c * c - a2) * cos2;
+    return c * (a2 + t) / (2.0 * a * std::sqrt(t * t * t));
+  }
+  double p1 = dphi(B, 0, x, true), p2 = dphi(B, 1, x, true), p3 = dphi(B, 2, x, true);
+  double H[9];
+  d2phi(B, H);
+  double p11 = H[0], p12 = H[1], p13 = H[2], p22 = H[4], p23 = H[5], p33 = H[8];
+  double s1 = p1 * p1, s2 = p2 * p2, s3 = p3 * p3;
+  double r = s1 * p22 - 2.0 * p1 * p2 * p12 + s2 * p11;
+  r += s1 * p33 - 2.0 * p1 * p3 * p13 + s3 * p11;
+  r += s2 * p33 - 2.0 * p2 * p3 * p23 + s3 * p22;
+  double d = s1 + s2 + s3;
+  return r / (2.0 * d * std::sqrt(d));
+}
+
+// surfc_force (surface.cpp:530-559).
+void surface_force(const hor_body& B, const double x0[3], const double n0[3], double f[3]) {
+  double H = mean_curvature(B, x0);
+  double zz = x0[2] - B.ctr[2];
+  double gamma = 1.0 + zz * zz;
+  double dgx = 0.0, dgy = 0.0, dgz = 2.0 * zz;
+  double dgn = dgx * n0[0] + dgy * n0[1] + dgz * n0[2];
+  double sg0 = dgx - dgn * n0[0], sg1 = dgy - dgn * n0[1], sg2 = dgz - dgn * n0[2];
+  gamma *= 2.0 * H;
+  f[0] = gamma * n0[0] - sg0;
+  f[1] = gamma * n0[1] - sg1;
+  f[2] = gamma * n0[2] - sg2;
+}
+
+// ---------------------------------------------------------------------------
+// k-d tree nearest-point search with the reference's tie rule (the first
+// index among equal distances wins, find_Nearest_grid surface.cpp:370-390).
+
+struct PointTree {
+  struct Node {
+    double lo[3], hi[3];
+    int begin, end;   // range in idx
+    int left, right;  // children (-1 for a leaf)
+  };
+  std::vector<Node> nodes;
+  std::vector<int> idx;
+  const double* x = nullptr;
+
+  void build(const double* pts, const std::vector<int>& ids) {
+    x = pts;
+    idx = ids;
+    nodes.clear();
+    nodes.reserve(2 * ids.size() / 8 + 2);
+    if (!idx.empty()) make(0, (int)idx.size());
+  }
+  int make(int b, int e) {
+    Node nd;
+    for (int k = 0; k < 3; k++) { nd.lo[k] = 1e300; nd.hi[k] = -1e300; }
+    for (int q = b; q < e; q++)
+      for (int k = 0; k < 3; k++) {
+        nd.lo[k] = std::min(nd.lo[k], x[3 * idx[q] + k]);
+        nd.hi[k] = std::max(nd.hi[k], x[3 * idx[q] + k]);
+      }
+    nd.begin = b;
+    nd.end = e;
+    nd.left = nd.right = -1;
+    int me = (int)nodes.size();
+    nodes.push_back(nd);
+    if (e - b > 16) {
+      int ax = 0;
+      for (int k = 1; k < 3; k++)
+        if (nd.hi[k] - nd.lo[k] > nd.hi[ax] - nd.lo[ax]) ax = k;
+      int mid = (b + e) / 2;
+      std::nth_element(idx.begin() + b, idx.begin() + mid, idx.begin() + e,
+                       [&](int a, int c) { return x[3 * a + ax] < x[3 * c + ax]; });
+      int l = make(b, mid);
+      int r = make(mid, e);
+      nodes[me].left = l;
+      nodes[me].right = r;
+    }
+    return me;
+  }
+  static double box_d2(const Node& nd, const double q[3]) {
+    double d2 = 0.0;
+    for (int k = 0; k < 3; k++) {
+      double g = std::max(std::max(nd.lo[k] - q[k], q[k] - nd.hi[k]), 0.0);
+      d2 += g * g;
+    }
+    return d2;
+  }
+  void search(int ni, const double q[3], double& bd, int& best) const {
+    const Node& nd = nodes[ni];
+    if (box_d2(nd, q) > bd) return;
+    if (nd.left < 0) {
+      for (int p = nd.begin; p < nd.end; p++) {
+        int i = idx[p];
+        const double* s = x + 3 * i;
+        double d = (q[0] - s[0]) * (q[0] - s[0]) + (q[1] - s[1]) * (q[1] - s[1]) +
+                   (q[2] - s[2]) * (q[2] - s[2]);
+        if (d < bd || (d == bd && best >= 0 && i < best)) { bd = d; best = i; }
+      }
+      return;
+    }
+    double dl = box_d2(nodes[nd.left], q), dr = box_d2(nodes[nd.right], q);
+    if (dl <= dr) { search(nd.left, q, bd, best); search(nd.right, q, bd, best); }
+    else { search(nd.right, q, bd, best); search(nd.left, q, bd, best); }
+  }
+  // Nearest point to q with squared distance < limit2; -1 if none.
+  int nearest(const double q[3], double limit2, double* d2out) const {
+    int best = -1;
+    double bd = limit2;
+    if (!nodes.empty()) search(0, q, bd, best);
+    if (d2out) *d2out = bd;
+    return best;
+  }
+};
+
+// Compute_nearest_data (surface.cpp:450-526).
+int nearest_data(const hor_body& B, double h, double DEL, int ns, const double* sx,
+                 const int* splane, int nt, const double* tx, int* nrst, double* x0,
+                 double* n0, double* b, int* i123, double* f0) {
+  PointTree grids[3];
+  for (int p = 0; p < 3; p++) {
+    std::vector<int> ids;
+    for (int i = 0; i < ns; i++)
+      if (splane[i] == p) ids.push_back(i);
+    grids[p].build(sx, ids);
+  }
+  int failures = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : failures)
+  for (int i = 0; i < nt; i++) {
+    const double* pt = tx + 3 * i;
+    if (f0) f0[3 * i] = f0[3 * i + 1] = f0[3 * i + 2] = 0.0;
+    int i1, i2, i3;
+    monge_patch(B, pt, i1, i2, i3);
+    int ni = grids[i3].nearest(pt, 100.0, nullptr);  // min_dist = 100 (surface.cpp:374)
+    nrst[i] = ni;
+    if (ni < 0) { failures++; continue; }
+    double xx0[3], nn0[3];
+    if (nearest_point(B, pt, sx + 3 * ni, xx0, nn0) == -1) { failures++; continue; }
+    for (int k = 0; k < 3; k++) {
+      x0[3 * i + k] = xx0[k];
+      n0[3 * i + k] = nn0[k];
+    }
+    i123[3 * i] = i1;
+    i123[3 * i + 1] = i2;
+    i123[3 * i + 2] = i3;
+    b[i] = (pt[0] - xx0[0]) * nn0[0] + (pt[1] - xx0[1]) * nn0[1] + (pt[2] - xx0[2]) * nn0[2];
+    if (std::fabs(b[i]) / DEL > 8.0) {
+      nrst[i] = 0;  // far sentinel (surface.cpp:500-503)
+      continue;
+    }
+    if (f0) surface_force(B, xx0, nn0, f0 + 3 * i);
+  }
+  return failures;
+}
+
+// Band targets: grid nodes with |b| <= band, the set find_nrst_surfc
+// (extend_to_grid/surface.cpp:303-392) tags with BAND1*h = band.  Candidates
+// are pre-filtered by the level set (|x|_abc - 1 bounds the distance by
+// a_min); the Newton projection (as find_Nearest) starts from the quadrature
+// point nearest to the node's radial projection onto the body.  Nodes come out
+// in (i, j, k) lexicographic order.
+int band_targets(const hor_body& B, const double* origin, double h, int ngrid, double band,
+                 int ns, const double* sx, std::vector<int>& ijk, std::vector<double>& X,
+                 std::vector<double>& b, std::vector<double>& X0, std::vector<double>& N0) {
+  const double tol = 1.e-12;
+  double amin = std::min(B.abc[0], std::min(B.abc[1], B.abc[2]));
+  double hi_bound = (1.0 + (band + 2 * h) / amin) * (1.0 + (band + 2 * h) / amin) - 1.0;
+  double lo_r = 1.0 - (band + 2 * h) / amin;
+  double lo_bound = lo_r > 0 ? lo_r * lo_r - 1.0 : -2.0;
+  std::vector<int> all(ns);
+  for (int i = 0; i < ns; i++) all[i] = i;
+  PointTree tree;
+  tree.build(sx, all);
+  int n1 = ngrid + 1;
+  std::vector<std::vector<int>> per_i(n1);
+  std::vector<std::vector<double>> per_d(n1);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int i = 0; i < n1; i++) {
+    for (int j = 0; j < n1; j++)
+      for (int k = 0; k < n1; k++) {
+        double pt[3] = {origin[0] + i * h, origin[1] + j * h, origin[2] + k * h};
+        double ph = phi(B, pt, true);
+        if (ph > hi_bound || ph < lo_bound) continue;
+        // radial projection onto the body: x_s = ctr + (pt - ctr)/|x_body|_abc
+        double s = std::sqrt(ph + 1.0);
+        if (!(s > 0)) continue;
+        double xs[3];
+        for (int d = 0; d < 3; d++) xs[d] = B.ctr[d] + (pt[d] - B.ctr[d]) / s;
+        int q = tree.nearest(xs, 1e300, nullptr);
+        if (q < 0) continue;
+        double x0[3], n0[3];
+        if (nearest_point(B, pt, sx + 3 * q, x0, n0) != 0) continue;
+        double bb = (pt[0] - x0[0]) * n0[0] + (pt[1] - x0[1]) * n0[1] + (pt[2] - x0[2]) * n0[2];
+        if (std::fabs(bb) <= band + tol) {
+          per_i[i].push_back(j);
+          per_i[i].push_back(k);
+          double rec[7] = {bb, x0[0], x0[1], x0[2], n0[0], n0[1], n0[2]};
+          per_d[i].insert(per_d[i].end(), rec, rec + 7);
+        }
+      }
+  }
+  for (int i = 0; i < n1; i++) {
+    size_t m = per_i[i].size() / 2;
+    for (size_t q = 0; q < m; q++) {
+      int j = per_i[i][2 * q], k = per_i[i][2 * q + 1];
+      ijk.push_back(i); ijk.push_back(j); ijk.push_back(k);
+      X.push_back(origin[0] + i * h);
+      X.push_back(origin[1] + j * h);
+      X.push_back(origin[2] + k * h);
+      const double* r = &per_d[i][7 * q];
+      b.push_back(r[0]);
+      X0.push_back(r[1]); X0.push_back(r[2]); X0.push_back(r[3]);
+      N0.push_back(r[4]); N0.push_back(r[5]); N0.push_back(r[6]);
+    }
+  }
+  return (int)b.size();
+}
+
+// ---------------------------------------------------------------------------
+// Translating ellipsoid.  The single-layer density that reproduces a rigid
+// translation U inside the body is f = F / (4 pi a b c) / sqrt(x^2/a^4 +
+// y^2/b^4 + z^2/c^4) (body frame) with the Oberbeck drag
+// F_i = 16 pi U_i / (chi + a_i^2 alpha_i), chi = int_0^inf dl/Delta,
+// alpha_i = int_0^inf dl/((a_i^2+l) Delta), Delta = sqrt(prod (a_k^2+l)).
+// The integrals are Carlson's symmetric forms: chi = 2 R_F(a^2,b^2,c^2),
+// alpha_1 = (2/3) R_D(b^2, c^2, a^2) (B. C. Carlson, Numer. Algorithms 10
+// (1995) 13-26, duplication algorithm).  For a prolate spheroid this is the
+// reference's stokeslet_density (extend_to_grid/utilities.cpp:267-285).
+
+static double carlson_rf(double x, double y, double z) {
+  for (int it = 0; it < 100; it++) {
+    double sx = std::sqrt(x), sy = std::sqrt(y), sz = std::sqrt(z);
+    double l = sx * sy + sx * sz + sy * sz;
+    x = 0.25 * (x + l); y = 0.25 * (y + l); z = 0.25 * (z + l);
+    double m = (x + y + z) / 3.0;
+    double dx = 1 - x / m, dy = 1 - y / m, dz = 1 - z / m;
+    double e = std::max(std::fabs(dx), std::max(std::fabs(dy), std::fabs(dz)));
+    if (e < 1e-4) {
+      double e2 = dx * dy - dz * dz, e3 = dx * dy * dz;
+      return (1 - e2 / 10 + e3 / 14 + e2 * e2 / 24 - 3 * e2 * e3 / 44) / std::sqrt(m);
+    }
+  }
+  return NAN;
+}
+
+static double carlson_rd(double x, double y, double z) {
+  double sum = 0.0, fac = 1.0;
+  for (int it = 0; it < 100; it++) {
+    double sx = std::sqrt(x), sy = std::sqrt(y), sz = std::sqrt(z);
+    double l = sx * sy + sx * sz + sy * sz;
+    sum += fac / (sz * (z + l));
+    fac *= 0.25;
+    x = 0.25 * (x + l); y = 0.25 * (y + l); z = 0.25 * (z + l);
+    double m = (x + y + 3 * z) / 5.0;
+    double dx = 1 - x / m, dy = 1 - y / m, dz = 1 - z / m;
+    double e = std::max(std::fabs(dx), std::max(std::fabs(dy), std::fabs(dz)));
+    if (e < 1e-4) {
+      double ea = dx * dy, eb = dz * dz, ec = ea - eb, ed = ea - 6 * eb, ee = ed + ec + ec;
+      double s = 1 + ed * (-3.0 / 14 + 9.0 / 88 * ed - 4.5 / 26 * dz * ee) +
+                 dz * (ee / 6 + dz * (-9.0 / 22 * ec + 3.0 / 26 * dz * ea));
+      return 3 * sum + fac * s / (m * std::sqrt(m));
+    }
+  }
+  return NAN;
+}
+
+void traction_translating(const hor_body& B, const double U[3], int n, const double* x, double* f) {
+  double a2[3] = {B.abc[0] * B.abc[0], B.abc[1] * B.abc[1], B.abc[2] * B.abc[2]};
+  double chi = 2.0 * carlson_rf(a2[0], a2[1], a2[2]);
+  double alpha[3] = {2.0 / 3.0 * carlson_rd(a2[1], a2[2], a2[0]),
+                     2.0 / 3.0 * carlson_rd(a2[2], a2[0], a2[1]),
+                     2.0 / 3.0 * carlson_rd(a2[0], a2[1], a2[2])};
+  // U in the body frame: U_b = R U (R maps global to body directions).
+  double Ub[3];
+  if (B.rotated) {
+    for (int i = 0; i < 3; i++) Ub[i] = B.R[3 * i] * U[0] + B.R[3 * i + 1] * U[1] + B.R[3 * i + 2] * U[2];
+  } else {
+    Ub[0] = U[0]; Ub[1] = U[1]; Ub[2] = U[2];
+  }
+  double Fb[3];
+  for (int i = 0; i < 3; i++) Fb[i] = 16.0 * kPi * Ub[i] / (chi + a2[i] * alpha[i]);
+  double F[3];
+  if (B.rotated) {
+    for (int i = 0; i < 3; i++) F[i] = B.R[i] * Fb[0] + B.R[3 + i] * Fb[1] + B.R[6 + i] * Fb[2];
+  } else {
+    F[0] = Fb[0]; F[1] = Fb[1]; F[2] = Fb[2];
+  }
+  double s = 1.0 / (4.0 * kPi * B.abc[0] * B.abc[1] * B.abc[2]);
+  for (int p = 0; p < n; p++) {
+    double u[3];
+    to_body(B, x + 3 * p, true, u);
+    double q = u[0] * u[0] / (a2[0] * a2[0]) + u[1] * u[1] / (a2[1] * a2[1]) + u[2] * u[2] / (a2[2] * a2[2]);
+    double w = s / std::sqrt(q);
+    for (int k = 0; k < 3; k++) f[3 * p + k] = F[k] * w;
+  }
+}
+
+// SL_exact (extend_to_grid/utilities.cpp:360-411): exterior velocity of a
+// prolate spheroid (semi-axes a > b = c along body x) translating with
+// U = (1,0,0); inside and on the body the velocity is U.
+void spheroid_exact(const hor_body& B, int n, const double* xin, double* uout) {
+  double a = B.abc[0], bb = B.abc[1];
+  double a2 = a * a, b2 = bb * bb;
+  double c = std::sqrt(a2 - b2);
+  double e = c / a;
+  double Le = std::log((1.0 + e) / (1.0 - e));
+  double a1 = e * e / ((1.0 + e * e) * Le - 2.0 * e);
+  double b1 = a1 * (1.0 - e * e) / (2.0 * e * e);
+  for (int p = 0; p < n; p++) {
+    double u[3];
+    to_body(B, xin + 3 * p, true, u);
+    double x = u[0], y = u[1], z = u[2];
+    double r2 = y * y + z * z;
+    double R1 = std::sqrt((x + c) * (x + c) + r2);
+    double R2 = std::sqrt((x - c) * (x - c) + r2);
+    double R1c = R1 - (x + c), R2c = R2 - (x - c);
+    double B10;
+    if (r2 < 1.e-14) B10 = (x < 0) ? std::log((x - c) / (x + c)) : std::log((x + c) / (x - c));
+    else B10 = std::log(R2c / R1c);
+    double B30 = (x + c) / R1 - (x - c) / R2;
+    double dB1dx = 1.0 / R1 - 1.0 / R2;
+    double dB1dyz = 0.0;
+    if (r2 > 1.e-14) dB1dyz = 1.0 / R2 / R2c - 1.0 / R1 / R1c;
+    double dB3dyz = 1.0 / R2 - 1.0 / R1 + x * dB1dyz;
+    double v[3];
+    v[0] = 2.0 * a1 * B10 - a1 * B30 + 2.0 * b1 * ((x - c) / R2 - (x + c) / R1 + B10 + x * dB1dx);
+    double Er = a1 * (1.0 / R2 - 1.0 / R1) + 2.0 * b1 * dB3dyz;
+    v[1] = Er * y;
+    v[2] = Er * z;
+    // body frame -> global frame
+    if (B.rotated) {
+      for (int i = 0; i < 3; i++) uout[3 * p + i] = B.R[i] * v[0] + B.R[3 + i] * v[1] + B.R[6 + i] * v[2];
+    } else {
+      for (int i = 0; i < 3; i++) uout[3 * p + i] = v[i];
+    }
+  }
+}
+
+}  // namespace hor
+
+// ---------------------------------------------------------------------------
+// C ABI wrappers
+
+extern "C" {
+
+int hor_body_init(hor_body* body, const double* abc, const double* ctr, const double* R) {
+  if (!body || !abc) return hor::set_error(HOR_E_ARG, "hor_body_init: null argument");
+  for (int k = 0; k < 3; k++) {
+    body->abc[k] = abc[k];
+    body->ctr[k] = ctr ? ctr[k] : 0.0;
+    if (!(abc[k] > 0)) return hor::set_error(HOR_E_ARG, "hor_body_init: semi-axes must be > 0");
+  }
+  body->rotated = R ? 1 : 0;
+  for (int k = 0; k < 9; k++) body->R[k] = R ? R[k] : ((k % 4 == 0) ? 1.0 : 0.0);
+  return HOR_OK;
+}
+
+int hor_surface_generate(const hor_body* body, double h, double pou_deg, int cap, int* n,
+                         double* x, double* nrml, double* area, int* plane) {
+  if (!body || !n || !(h > 0)) return hor::set_error(HOR_E_ARG, "hor_surface_generate: bad argument");
+  std::vector<double> X, Nr, Ar;
+  std::vector<int> Pl;
+  int cnt = hor::surface_generate(*body, h, pou_deg, X, Nr, Ar, Pl);
+  *n = cnt;
+  if (cap >= cnt && cnt > 0) {
+    if (!x || !nrml || !area || !plane) return hor::set_error(HOR_E_ARG, "hor_surface_generate: null output");
+    std::memcpy(x, X.data(), sizeof(double) * X.size());
+    std::memcpy(nrml, Nr.data(), sizeof(double) * Nr.size());
+    std::memcpy(area, Ar.data(), sizeof(double) * Ar.size());
+    std::memcpy(plane, Pl.data(), sizeof(int) * Pl.size());
+  }
+  return HOR_OK;
+}
+
+int hor_nearest_data(const hor_body* body, double h, double DEL, int ns, const double* sx,
+                     const int* splane, int nt, const double* tx, int* nrst, double* x0,
+                     double* n0, double* b, int* i123, double* f0) {
+  if (!body || !sx || !splane || !tx || !nrst || !x0 || !n0 || !b || !i123 || ns <= 0 || nt < 0)
+    return hor::set_error(HOR_E_ARG, "hor_nearest_data: bad argument");
+  int fails = hor::nearest_data(*body, h, DEL, ns, sx, splane, nt, tx, nrst, x0, n0, b, i123, f0);
+  if (fails) return hor::set_error(HOR_E_ARG, "hor_nearest_data: nearest point not found for %d targets", fails);
+  return HOR_OK;
+}
+
+int hor_surface_force(const hor_body* body, int n, const double* x, const double* nrml, double* f) {
+  if (!body || !x || !nrml || !f || n < 0) return hor::set_error(HOR_E_ARG, "hor_surface_force: bad argument");
+  for (int i = 0; i < n; i++) hor::surface_force(*body, x + 3 * i, nrml + 3 * i, f + 3 * i);
+  return HOR_OK;
+}
+
+int hor_band_targets(const hor_body* body, const double* origin, double h, int ngrid, double band,
+                     int ns, const double* sx, int cap, int* count, int* ijk, double* x,
+                     double* b, double* x0, double* n0) {
+  if (!body || !origin || !sx || !count || ns <= 0 || !(h > 0) || ngrid < 1)
+    return hor::set_error(HOR_E_ARG, "hor_band_targets: bad argument");
+  std::vector<int> IJK;
+  std::vector<double> X, Bv, X0, N0v;
+  int cnt = hor::band_targets(*body, origin, h, ngrid, band, ns, sx, IJK, X, Bv, X0, N0v);
+  *count = cnt;
+  if (cap >= cnt && cnt > 0) {
+    if (!ijk || !x || !b || !x0 || !n0) return hor::set_error(HOR_E_ARG, "hor_band_targets: null output");
+    std::memcpy(ijk, IJK.data(), sizeof(int) * IJK.size());
+    std::memcpy(x, X.data(), sizeof(double) * X.size());
+    std::memcpy(b, Bv.data(), sizeof(double) * Bv.size());
+    std::memcpy(x0, X0.data(), sizeof(double) * X0.size());
+    std::memcpy(n0, N0v.data(), sizeof(double) * N0v.size());
+  }
+  return HOR_OK;
+}
+
+int hor_traction_translating(const hor_body* body, const double* U, int n, const double* x, double* f) {
+  if (!body || !U || !x || !f || n < 0) return hor::set_error(HOR_E_ARG, "hor_traction_translating: bad argument");
+  hor::traction_translating(*body, U, n, x, f);
+  return HOR_OK;
+}
+
+int hor_spheroid_exact_velocity(const hor_body* body, int n, const double* x, double* u) {
+  if (!body || !x || !u || n < 0) return hor::set_error(HOR_E_ARG, "hor_spheroid_exact_velocity: bad argument");
+  if (!(body->abc[0] > body->abc[1] && body->abc[1] == body->abc[2]))
+    return hor::set_error(HOR_E_ARG, "hor_spheroid_exact_velocity: needs a prolate spheroid a > b = c");
+  hor::spheroid_exact(*body, n, x, u);
+  return HOR_OK;
+}
+
+}  // extern "C"

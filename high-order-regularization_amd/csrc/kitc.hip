@@ -1,0 +1,600 @@
+// KITC treecode on gfx950 (KITC.h, KITC.cpp).
+//
+// Set-up (geometry only, once per plan): the octree over the sources and
+// every target's interaction list are built on the host exactly as the
+// reference's recursive traversal decides them; the lists are transposed into
+// per-node work items (a node and up to 256 targets that visit it), and the
+// barycentric factors a1, a2, a3 and D of every source in every far-field
+// node are tabulated.
+//
+// Evaluation (per density, all on the device):
+//   1. k_moments:  proxy weights W[c][kk] = sum_s a1_i a2_j a3_k D A q_c over
+//                  node members, split into member chunks, summed in order;
+//   2. k_leaf:     regularized direct sums of a leaf's sources for its near
+//                  targets (same pair code and LDS tiling as the direct kernel);
+//   3. k_far:      plain Stokeslet / stresslet from the (P+1)^3 proxies with
+//                  target-side subtraction (Call_Treecode_SL/DL);
+//   4. k_tree_finalize: each target adds its list's slots in visiting order,
+//                  scales and adds the jump term.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "pairs.h"
+#include "plan.h"
+
+namespace hor {
+
+constexpr int kMomChunk = 1024;  // sources per moment work item
+
+struct TreePlan {
+  Tree T;
+  TargetLists L;
+  int np1 = 0, pflat = 0, ncomp = 0;
+  int nslot = 0;
+  // node data on the device
+  DBuf<int> node_lo, node_hi, leaf_sub_off, far_idx;
+  DBuf<double> cheb, leaf_box;
+  // near work items
+  int n_near_items = 0;
+  DBuf<int> near_item_node, near_item_off, near_item_cnt, near_tgt, near_slot;
+  // far work items
+  int n_far_items = 0;
+  DBuf<int> far_item_node, far_item_off, far_item_cnt, far_tgt, far_slot;
+  // moments
+  int nfar_nodes = 0, n_mom_items = 0;
+  DBuf<int> mom_item_node, mom_item_s0, mom_item_s1, mom_node_first, mom_node_nitems, fac_off;
+  DBuf<double> fac;          // per far node: A1 (np1*n), A2, A3, D (n)
+  DBuf<double> mom_part, mom;
+  // reduction
+  DBuf<int> slot_start;      // caller order, nt+1
+  DBuf<double> slots;        // 3*nslot
+};
+
+// ---------------------------------------------------------------------------
+// Device kernels
+
+struct MomArgs {
+  int np1, pflat, ncomp, nsp;
+  const int *item_node, *item_s0, *item_s1;  // per item: node, member range [s0, s1) (tree slots)
+  const int *node_lo, *fac_off, *far_idx;
+  const double* fac;
+  const double *sn, *sa, *sd;                // tree-order sources (stride nsp)
+  double* part;                              // n_items * ncomp * pflat
+};
+
+// Panel_Moments_SL / Panel_Moments_DL (KITC.cpp:351-668), over one member chunk.
+template <bool DL>
+__global__ __launch_bounds__(kBlock) void k_moments(MomArgs A) {
+  constexpr int NC = DL ? 12 : 6;
+  const int item = blockIdx.x;
+  const int kk = blockIdx.y * kBlock + threadIdx.x;
+  const int node = A.item_node[item];
+  const int s0 = A.item_s0[item], s1 = A.item_s1[item];
+  const int lo = A.node_lo[node];
+  const int np1 = A.np1;
+  const int nmem = A.fac_off[A.far_idx[node] + 1] - A.fac_off[A.far_idx[node]];
+  const int nm = nmem / (3 * np1 + 1);
+  const double* F = A.fac + A.fac_off[A.far_idx[node]];
+  const double* A1 = F;
+  const double* A2 = F + (size_t)np1 * nm;
+  const double* A3 = F + (size_t)2 * np1 * nm;
+  const double* D = F + (size_t)3 * np1 * nm;
+  bool act = kk < A.pflat;
+  int i = 0, j = 0, k = 0;
+  if (act) {
+    i = kk / (np1 * np1);
+    j = (kk / np1) % np1;
+    k = kk % np1;
+  }
+  double sum[NC];
+#pragma unroll
+  for (int c = 0; c < NC; c++) sum[c] = 0.0;
+  for (int sl = s0; sl < s1; sl++) {
+    const int m = sl - lo;  // member index within the node
+    const int p = sl;       // tree slot
+    double w = A1[(size_t)i * nm + m] * A2[(size_t)j * nm + m] * A3[(size_t)k * nm + m] * D[m];
+    w *= A.sa[p];
+    double nx = A.sn[p], ny = A.sn[A.nsp + p], nz = A.sn[2 * A.nsp + p];
+    double l0 = A.sd[p], l1 = A.sd[A.nsp + p], l2 = A.sd[2 * A.nsp + p];
+    if (!DL) {
+      sum[0] += w * l0; sum[1] += w * l1; sum[2] += w * l2;
+      sum[3] += w * nx; sum[4] += w * ny; sum[5] += w * nz;
+    } else {
+      double wl[3] = {w * l0, w * l1, w * l2};
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        sum[3 * c] += wl[c] * nx;
+        sum[3 * c + 1] += wl[c] * ny;
+        sum[3 * c + 2] += wl[c] * nz;
+      }
+      sum[9] += w * nx; sum[10] += w * ny; sum[11] += w * nz;
+    }
+  }
+  if (act) {
+    double* P = A.part + (size_t)item * NC * A.pflat;
+#pragma unroll
+    for (int c = 0; c < NC; c++) P[(size_t)c * A.pflat + kk] = sum[c];
+  }
+}
+
+// Sums the member-chunk partials of each far node in chunk order.
+__global__ void k_moments_reduce(int nfar, int ncomp, int pflat, const int* node_first, const int* node_nitems,
+                                 const double* part, double* mom) {
+  int f = blockIdx.x;
+  int total = ncomp * pflat;
+  for (int e = threadIdx.x; e < total; e += blockDim.x) {
+    double s = 0.0;
+    int it0 = node_first[f], nit = node_nitems[f];
+    for (int q = 0; q < nit; q++) s += part[(size_t)(it0 + q) * total + e];
+    mom[(size_t)f * total + e] = s;
+  }
+}
+
+struct LeafArgs {
+  DirectArgs D;
+  const int *item_node, *item_off, *item_cnt, *tgt, *slot;
+  const int *node_lo, *node_hi, *leaf_sub_off;
+  double* slots;
+  int nslot;
+};
+
+__device__ __forceinline__ double wave_min(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o));
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+  return v;
+}
+
+// Leaf direct sums (Call_Ds_*, KITC.cpp:807-1062) for up to 256 near targets.
+template <int KIND, bool MIXED>
+__global__ __launch_bounds__(kBlock) void k_leaf(LeafArgs A) {
+  __shared__ SrcRec tile[kBlock];
+  __shared__ SrcRec32 tile32[MIXED ? kBlock : 1];
+  const int tid = threadIdx.x;
+  const int item = blockIdx.x;
+  const int node = A.item_node[item];
+  const int off = A.item_off[item], cnt = A.item_cnt[item];
+  const bool act = tid < cnt;
+  const int li = off + (act ? tid : 0);
+  const int i = A.tgt[li];
+  Tgt t;
+  load_target<KIND>(A.D, i, t);
+  // wave bounding box and class
+  double wb[6];
+  wb[0] = wave_min(t.px); wb[1] = wave_min(t.py); wb[2] = wave_min(t.pz);
+  wb[3] = wave_max(t.px); wb[4] = wave_max(t.py); wb[5] = wave_max(t.pz);
+  const bool off_kind = (KIND == HOR_SL_OFF || KIND == HOR_DL_OFF);
+  const bool all_far = off_kind && !__any(t.nearp ? 1 : 0);
+  float pxf = (float)t.px, pyf = (float)t.py, pzf = (float)t.pz;
+  double u[3] = {0.0, 0.0, 0.0};
+  const int lo = A.node_lo[node], hi = A.node_hi[node];
+  const int sub0 = A.leaf_sub_off[node];
+  for (int base = lo; base <= hi; base += kBlock) {
+    const int len = min(kBlock, hi + 1 - base);
+    __syncthreads();
+    if (tid < len) {
+      int j = base + tid;
+      const DirectArgs& D = A.D;
+      SrcRec r;
+      r.x = D.sx[j]; r.y = D.sx[D.nsp + j]; r.z = D.sx[2 * D.nsp + j]; r.a = D.sa[j];
+      r.nx = D.sn[j]; r.ny = D.sn[D.nsp + j]; r.nz = D.sn[2 * D.nsp + j]; r.pad0 = 0;
+      r.d0 = D.sd[j]; r.d1 = D.sd[D.nsp + j]; r.d2 = D.sd[2 * D.nsp + j]; r.pad1 = 0;
+      tile[tid] = r;
+      if (MIXED) {
+        SrcRec32 q;
+        q.x = (float)r.x; q.y = (float)r.y; q.z = (float)r.z; q.a = (float)r.a;
+        q.nx = (float)r.nx; q.ny = (float)r.ny; q.nz = (float)r.nz; q.pad0 = 0;
+        q.d0 = (float)r.d0; q.d1 = (float)r.d1; q.d2 = (float)r.d2; q.pad1 = 0;
+        tile32[tid] = q;
+      }
+    }
+    __syncthreads();
+    tile_sum<KIND, MIXED>(A.D, tile, tile32, len, sub0 + (base - lo) / kWave, wb, all_far, t, pxf, pyf,
+                          pzf, u);
+  }
+  if (act) {
+    int s = A.slot[li];
+    A.slots[s] = u[0];
+    A.slots[A.nslot + s] = u[1];
+    A.slots[2 * A.nslot + s] = u[2];
+  }
+}
+
+struct FarArgs {
+  int np1, pflat, ntp, nslot;
+  const int *item_node, *item_off, *item_cnt, *tgt, *slot, *far_idx;
+  const double *cheb, *mom, *tx, *tsub2;
+  double* slots;
+};
+
+// Call_Treecode_SL / Call_Treecode_DL (KITC.cpp:671-777).
+template <bool DL, bool MIXED>
+__global__ __launch_bounds__(kBlock) void k_far(FarArgs A) {
+  const int tid = threadIdx.x;
+  const int item = blockIdx.x;
+  const int node = A.item_node[item];
+  const int off = A.item_off[item], cnt = A.item_cnt[item];
+  const bool act = tid < cnt;
+  const int li = off + (act ? tid : 0);
+  const int i = A.tgt[li];
+  const int np1 = A.np1, pflat = A.pflat;
+  const double px = A.tx[i], py = A.tx[A.ntp + i], pz = A.tx[2 * A.ntp + i];
+  const double* t1 = A.cheb + (size_t)3 * np1 * node;
+  const double* t2 = t1 + np1;
+  const double* t3 = t2 + np1;
+  constexpr int NC = DL ? 12 : 6;
+  const double* M = A.mom + (size_t)A.far_idx[node] * NC * pflat;
+  double u0 = 0.0, u1 = 0.0, u2 = 0.0;
+  if (!DL) {
+    const double sub = A.tsub2[i];
+    int kk = 0;
+    for (int a = 0; a < np1; a++) {
+      double d0 = px - t1[a], ti = d0 * d0;
+      for (int b = 0; b < np1; b++) {
+        double d1 = py - t2[b], tj = d1 * d1;
+        for (int c = 0; c < np1; c++, kk++) {
+          double d2 = pz - t3[c];
+          double m0 = M[kk] - sub * M[3 * pflat + kk];
+          double m1 = M[pflat + kk] - sub * M[4 * pflat + kk];
+          double m2 = M[2 * pflat + kk] - sub * M[5 * pflat + kk];
+          if (MIXED) {
+            float fd0 = (float)d0, fd1 = (float)d1, fd2 = (float)d2;
+            float r2 = fd0 * fd0 + fd1 * fd1 + fd2 * fd2;
+            float ri = rsqrt32(r2);
+            float f0 = (float)m0, f1 = (float)m1, f2 = (float)m2;
+            float tmp = (f0 * fd0 + f1 * fd1 + f2 * fd2) * (ri * ri * ri);
+            u0 += (double)(f0 * ri + tmp * fd0);
+            u1 += (double)(f1 * ri + tmp * fd1);
+            u2 += (double)(f2 * ri + tmp * fd2);
+          } else {
+            double r2 = ti + tj + d2 * d2;
+            double ri = rsqrt64(r2);
+            double tmp = (m0 * d0 + m1 * d1 + m2 * d2) * (ri * ri * ri);
+            u0 += m0 * ri + tmp * d0;
+            u1 += m1 * ri + tmp * d1;
+            u2 += m2 * ri + tmp * d2;
+          }
+        }
+      }
+    }
+  } else {
+    const double g0x = A.tsub2[i], g0y = A.tsub2[A.ntp + i], g0z = A.tsub2[2 * A.ntp + i];
+    const double g0[3] = {g0x, g0y, g0z};
+    int kk = 0;
+    for (int a = 0; a < np1; a++) {
+      double d0 = px - t1[a], ti = d0 * d0;
+      for (int b = 0; b < np1; b++) {
+        double d1 = py - t2[b], tj = d1 * d1;
+        for (int c = 0; c < np1; c++, kk++) {
+          double d2 = pz - t3[c];
+          double wn0 = M[9 * pflat + kk], wn1 = M[10 * pflat + kk], wn2 = M[11 * pflat + kk];
+          double tmp[3];
+#pragma unroll
+          for (int l = 0; l < 3; l++) {
+            double a1 = M[(3 * l) * pflat + kk] - g0[l] * wn0;
+            double a2 = M[(3 * l + 1) * pflat + kk] - g0[l] * wn1;
+            double a3 = M[(3 * l + 2) * pflat + kk] - g0[l] * wn2;
+            tmp[l] = a1 * d0 + a2 * d1 + a3 * d2;
+          }
+          if (MIXED) {
+            float fd0 = (float)d0, fd1 = (float)d1, fd2 = (float)d2;
+            float r2 = fd0 * fd0 + fd1 * fd1 + fd2 * fd2;
+            float ri = rsqrt32(r2), ri2 = ri * ri;
+            float kern = ((float)tmp[0] * fd0 + (float)tmp[1] * fd1 + (float)tmp[2] * fd2) * (ri2 * ri2 * ri);
+            u0 += (double)(fd0 * kern);
+            u1 += (double)(fd1 * kern);
+            u2 += (double)(fd2 * kern);
+          } else {
+            double r2 = ti + tj + d2 * d2;
+            double ri = rsqrt64(r2), ri2 = ri * ri;
+            double kern = (tmp[0] * d0 + tmp[1] * d1 + tmp[2] * d2) * (ri2 * ri2 * ri);
+            u0 += d0 * kern;
+            u1 += d1 * kern;
+            u2 += d2 * kern;
+          }
+        }
+      }
+    }
+  }
+  if (act) {
+    int s = A.slot[li];
+    A.slots[s] = u0;
+    A.slots[A.nslot + s] = u1;
+    A.slots[2 * A.nslot + s] = u2;
+  }
+}
+
+// Each target sums its slots in visiting order, then v = scale*sum + xi*g0
+// combined into out as the direct finalize does.
+__global__ void k_tree_finalize(int nt, int ntp, int nslot, const int* tperm, const int* slot_start,
+                                const double* slots, double scale, const double* xi, const double* g0,
+                                double* out, const double* base, double coef, int accumulate) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nt) return;
+  int o = tperm[i];
+  int a = slot_start[o], b = slot_start[o + 1];
+  for (int k = 0; k < 3; k++) {
+    double s = 0.0;
+    for (int q = a; q < b; q++) s += slots[(size_t)k * nslot + q];
+    double v = s * scale;
+    if (xi) v += xi[i] * g0[(size_t)k * ntp + i];
+    double w = coef * v;
+    if (accumulate) out[3 * o + k] += w;
+    else if (base) out[3 * o + k] = base[3 * o + k] + w;
+    else out[3 * o + k] = w;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host set-up
+
+// Barycentric factors of one source for one node's Chebyshev points
+// (Panel_Moments_*, KITC.cpp:398-459): a_j = w_j/(y - t_j), snapped to the
+// node when |y - t_j| < 1e-7, D = 1/(sum a1 sum a2 sum a3).
+static void bary_factors(int P, const double* t1, const double* t2, const double* t3, double x, double y,
+                         double z, double* a1, double* a2, double* a3, double* D) {
+  int np1 = P + 1;
+  double S[3] = {0, 0, 0};
+  int flag[3] = {-1, -1, -1};
+  const double* T[3] = {t1, t2, t3};
+  double v[3] = {x, y, z};
+  double* out[3] = {a1, a2, a3};
+  for (int d = 0; d < 3; d++)
+    for (int j = 0; j < np1; j++) out[d][j] = 0.0;
+  for (int j = 0; j < np1; j++) {
+    double dj = (j == 0 || j == P) ? 0.5 : 1.0;
+    double w = ((j % 2 == 0) ? 1 : -1) * dj;
+    for (int d = 0; d < 3; d++) {
+      if (std::fabs(v[d] - T[d][j]) < 1.e-07) {
+        flag[d] = j;
+      } else {
+        double tmp = w / (v[d] - T[d][j]);
+        out[d][j] = tmp;
+        S[d] += tmp;
+      }
+    }
+  }
+  for (int d = 0; d < 3; d++)
+    if (flag[d] > -1) {
+      S[d] = 1.0;
+      for (int j = 0; j < np1; j++) out[d][j] = 0.0;
+      out[d][flag[d]] = 1.0;
+    }
+  *D = 1.0 / (S[0] * S[1] * S[2]);
+}
+
+int tree_plan_create(hor_plan* P, const double* sx, const double* sn, const double* sa, const double* tx,
+                     const double* tb, const double* tx0, const double* tn0, const double* ctr,
+                     const hor_tree_params& tp) {
+  hipStream_t s = P->ctx->stream;
+  P->tree.reset(new TreePlan());
+  TreePlan& TP = *P->tree;
+  Tree& T = TP.T;
+  tree_build(T, P->ns, sx, ctr, tp.P, tp.N0, tp.theta);
+  TP.np1 = tp.P + 1;
+  TP.pflat = TP.np1 * TP.np1 * TP.np1;
+  bool dl = (P->kind == HOR_DL_ON || P->kind == HOR_DL_OFF);
+  TP.ncomp = dl ? 12 : 6;
+  // sources in tree order; targets sorted spatially (the class rule of the
+  // treecode's leaf sums: |b|/delta > 8, KITC.cpp:856, 970)
+  HOR_TRY(setup_sources(P, T.perm, sx, sn, sa));
+  HOR_TRY(setup_targets(P, tx, tb, tx0, tn0, sx));
+  const int nt = P->nt, nn = (int)T.nodes.size();
+  tree_lists(T, nt, tx, TP.L);
+  TP.nslot = TP.L.start[nt];
+  // node arrays, leaf subtile boxes (64-source subtiles starting at each node's lo)
+  std::vector<int> lo(nn), hi(nn), lso(nn + 1, 0);
+  for (int q = 0; q < nn; q++) {
+    lo[q] = T.nodes[q].lo;
+    hi[q] = T.nodes[q].hi;
+    lso[q + 1] = lso[q] + (hi[q] - lo[q] + kWave) / kWave;
+  }
+  std::vector<double> lbox(6 * (size_t)std::max(lso[nn], 1), 0.0);
+  for (int q = 0; q < nn; q++)
+    for (int b = 0; b < lso[q + 1] - lso[q]; b++) {
+      double l[3] = {1e300, 1e300, 1e300}, h[3] = {-1e300, -1e300, -1e300};
+      for (int p = lo[q] + b * kWave; p <= std::min(hi[q], lo[q] + (b + 1) * kWave - 1); p++) {
+        int o = T.perm[p];
+        for (int k = 0; k < 3; k++) {
+          l[k] = std::min(l[k], sx[3 * o + k]);
+          h[k] = std::max(h[k], sx[3 * o + k]);
+        }
+      }
+      for (int k = 0; k < 3; k++) { lbox[6 * (lso[q] + b) + k] = l[k]; lbox[6 * (lso[q] + b) + 3 + k] = h[k]; }
+    }
+  // transpose the lists into per-node work items, targets in sorted order
+  std::vector<std::vector<int>> ntg(nn), nsl(nn), ftg(nn), fsl(nn);
+  for (int i = 0; i < nt; i++) {
+    int o = P->tperm_h[i];
+    for (int q = TP.L.start[o]; q < TP.L.start[o + 1]; q++) {
+      int node = TP.L.node[q];
+      if (TP.L.far[q]) { ftg[node].push_back(i); fsl[node].push_back(q); }
+      else { ntg[node].push_back(i); nsl[node].push_back(q); }
+    }
+  }
+  auto items = [&](std::vector<std::vector<int>>& tg, std::vector<std::vector<int>>& sl, std::vector<int>& inode,
+                   std::vector<int>& ioff, std::vector<int>& icnt, std::vector<int>& atg, std::vector<int>& asl) {
+    for (int q = 0; q < nn; q++) {
+      int m = (int)tg[q].size();
+      for (int a = 0; a < m; a += kBlock) {
+        inode.push_back(q);
+        ioff.push_back((int)atg.size() + a);
+        icnt.push_back(std::min(kBlock, m - a));
+      }
+      atg.insert(atg.end(), tg[q].begin(), tg[q].end());
+      asl.insert(asl.end(), sl[q].begin(), sl[q].end());
+    }
+  };
+  std::vector<int> ni_node, ni_off, ni_cnt, n_tg, n_sl, fi_node, fi_off, fi_cnt, f_tg, f_sl;
+  items(ntg, nsl, ni_node, ni_off, ni_cnt, n_tg, n_sl);
+  items(ftg, fsl, fi_node, fi_off, fi_cnt, f_tg, f_sl);
+  TP.n_near_items = (int)ni_node.size();
+  TP.n_far_items = (int)fi_node.size();
+  // far-field nodes: moments and barycentric factors
+  std::vector<int> far_idx(nn + 1, -1), fac_off;
+  std::vector<int> far_nodes;
+  for (int q = 0; q < nn; q++)
+    if (!ftg[q].empty()) { far_idx[q] = (int)far_nodes.size(); far_nodes.push_back(q); }
+  TP.nfar_nodes = (int)far_nodes.size();
+  fac_off.assign(TP.nfar_nodes + 1, 0);
+  for (int f = 0; f < TP.nfar_nodes; f++) {
+    int q = far_nodes[f];
+    fac_off[f + 1] = fac_off[f] + (hi[q] - lo[q] + 1) * (3 * TP.np1 + 1);
+  }
+  std::vector<double> fac(std::max<size_t>(fac_off[TP.nfar_nodes], 1), 0.0);
+  std::vector<int> mi_node, mi_s0, mi_s1, m_first(TP.nfar_nodes), m_nit(TP.nfar_nodes);
+  double near_pairs = 0, moment_terms = 0;
+  for (int f = 0; f < TP.nfar_nodes; f++) {
+    int q = far_nodes[f], nm = hi[q] - lo[q] + 1, np1 = TP.np1;
+    double* F = &fac[fac_off[f]];
+    const double* cq = &T.cheb[3 * (size_t)np1 * q];
+    std::vector<double> a1(np1), a2(np1), a3(np1);
+    for (int m = 0; m < nm; m++) {
+      int o = T.perm[lo[q] + m];
+      double Dv;
+      bary_factors(tp.P, cq, cq + np1, cq + 2 * np1, sx[3 * o], sx[3 * o + 1], sx[3 * o + 2], a1.data(),
+                   a2.data(), a3.data(), &Dv);
+      for (int j = 0; j < np1; j++) {
+        F[(size_t)j * nm + m] = a1[j];
+        F[(size_t)(np1 + j) * nm + m] = a2[j];
+        F[(size_t)(2 * np1 + j) * nm + m] = a3[j];
+      }
+      F[(size_t)3 * np1 * nm + m] = Dv;
+    }
+    m_first[f] = (int)mi_node.size();
+    for (int a = lo[q]; a <= hi[q]; a += kMomChunk) {
+      mi_node.push_back(q);
+      mi_s0.push_back(a);
+      mi_s1.push_back(std::min(hi[q] + 1, a + kMomChunk));
+    }
+    m_nit[f] = (int)mi_node.size() - m_first[f];
+    moment_terms += (double)nm * TP.pflat;
+  }
+  TP.n_mom_items = (int)mi_node.size();
+  for (int q = 0; q < nn; q++) near_pairs += (double)ntg[q].size() * (hi[q] - lo[q] + 1);
+  P->near_pairs = near_pairs;
+  P->proxy_pairs = (double)f_tg.size() * TP.pflat;
+  P->moment_terms = moment_terms;
+  P->direct_pairs = (double)P->ns * P->nt;
+  // uploads
+  HOR_TRY(TP.node_lo.upload(lo, s));
+  HOR_TRY(TP.node_hi.upload(hi, s));
+  HOR_TRY(TP.leaf_sub_off.upload(lso, s));
+  HOR_TRY(TP.leaf_box.upload(lbox, s));
+  HOR_TRY(TP.far_idx.upload(far_idx, s));
+  HOR_TRY(TP.cheb.upload(T.cheb, s));
+  HOR_TRY(TP.near_item_node.upload(ni_node, s));
+  HOR_TRY(TP.near_item_off.upload(ni_off, s));
+  HOR_TRY(TP.near_item_cnt.upload(ni_cnt, s));
+  HOR_TRY(TP.near_tgt.upload(n_tg, s));
+  HOR_TRY(TP.near_slot.upload(n_sl, s));
+  HOR_TRY(TP.far_item_node.upload(fi_node, s));
+  HOR_TRY(TP.far_item_off.upload(fi_off, s));
+  HOR_TRY(TP.far_item_cnt.upload(fi_cnt, s));
+  HOR_TRY(TP.far_tgt.upload(f_tg, s));
+  HOR_TRY(TP.far_slot.upload(f_sl, s));
+  HOR_TRY(TP.mom_item_node.upload(mi_node, s));
+  HOR_TRY(TP.mom_item_s0.upload(mi_s0, s));
+  HOR_TRY(TP.mom_item_s1.upload(mi_s1, s));
+  HOR_TRY(TP.mom_node_first.upload(m_first, s));
+  HOR_TRY(TP.mom_node_nitems.upload(m_nit, s));
+  HOR_TRY(TP.fac_off.upload(fac_off, s));
+  HOR_TRY(TP.fac.upload(fac, s));
+  HOR_TRY(TP.mom_part.alloc((size_t)std::max(TP.n_mom_items, 1) * TP.ncomp * TP.pflat));
+  HOR_TRY(TP.mom.alloc((size_t)std::max(TP.nfar_nodes, 1) * TP.ncomp * TP.pflat));
+  HOR_TRY(TP.slot_start.upload(TP.L.start, s));
+  HOR_TRY(TP.slots.alloc(3 * (size_t)std::max(TP.nslot, 1)));
+  // xi of the treecode off-surface double layer (set by setup_targets for
+  // method == HOR_TREE) uses the given g0 of every target.
+  return HOR_OK;
+}
+
+template <int KIND>
+static void launch_leaf(const LeafArgs& A, bool mixed, int nitems, hipStream_t s) {
+  if (mixed) hipLaunchKernelGGL((k_leaf<KIND, true>), dim3(nitems), dim3(kBlock), 0, s, A);
+  else hipLaunchKernelGGL((k_leaf<KIND, false>), dim3(nitems), dim3(kBlock), 0, s, A);
+}
+
+int tree_plan_eval(hor_plan* P, const double* dens_dev, const double* d0_dev, double* out_dev, const double* base,
+                   double coef, int accumulate) {
+  hor_ctx* c = P->ctx;
+  hipStream_t s = c->stream;
+  TreePlan& TP = *P->tree;
+  const bool mixed = P->precision == HOR_MIXED;
+  const bool dl = (P->kind == HOR_DL_ON || P->kind == HOR_DL_OFF);
+  // density in tree order; per-target subtraction data (leaf and proxy variants)
+  HOR_TRY(launch_gather3(dens_dev, P->sperm.p, P->ns, P->nsp, P->sd.p, s));
+  HOR_TRY(prepare_targets(P, dens_dev, d0_dev));
+  if (P->kind == HOR_SL_OFF)
+    HOR_TRY(launch_target_f0n0_all(P->nt, P->ntp, P->tperm.p, d0_dev, P->n0_orig.p, P->tsub2.p, s));
+  else if (P->kind == HOR_DL_OFF)
+    HOR_TRY(launch_target_sub(4, P->nt, P->ntp, P->tperm.p, P->tnear.p, d0_dev, nullptr, nullptr, nullptr,
+                              P->tsub2.p, s));
+  else
+    HOR_HIP_TRY(hipMemcpyAsync(P->tsub2.p, P->tsub.p, sizeof(double) * 3 * P->ntp, hipMemcpyDeviceToDevice, s));
+  HOR_HIP_TRY(hipEventRecord(c->ev[2], s));
+  // 1. moments
+  if (TP.n_mom_items > 0) {
+    MomArgs M;
+    M.np1 = TP.np1; M.pflat = TP.pflat; M.ncomp = TP.ncomp; M.nsp = P->nsp;
+    M.item_node = TP.mom_item_node.p; M.item_s0 = TP.mom_item_s0.p; M.item_s1 = TP.mom_item_s1.p;
+    M.node_lo = TP.node_lo.p; M.fac_off = TP.fac_off.p; M.far_idx = TP.far_idx.p; M.fac = TP.fac.p;
+    M.sn = P->sn.p; M.sa = P->sa.p; M.sd = P->sd.p; M.part = TP.mom_part.p;
+    dim3 grid(TP.n_mom_items, (TP.pflat + kBlock - 1) / kBlock);
+    if (dl) hipLaunchKernelGGL((k_moments<true>), grid, dim3(kBlock), 0, s, M);
+    else hipLaunchKernelGGL((k_moments<false>), grid, dim3(kBlock), 0, s, M);
+    hipLaunchKernelGGL(k_moments_reduce, dim3(TP.nfar_nodes), dim3(kBlock), 0, s, TP.nfar_nodes, TP.ncomp,
+                       TP.pflat, TP.mom_node_first.p, TP.mom_node_nitems.p, TP.mom_part.p, TP.mom.p);
+    HOR_HIP_TRY(hipGetLastError());
+  }
+  // 2. near leaves
+  if (TP.n_near_items > 0) {
+    LeafArgs A;
+    A.D = direct_args(P);
+    A.D.sub_box = TP.leaf_box.p;
+    A.item_node = TP.near_item_node.p; A.item_off = TP.near_item_off.p; A.item_cnt = TP.near_item_cnt.p;
+    A.tgt = TP.near_tgt.p; A.slot = TP.near_slot.p;
+    A.node_lo = TP.node_lo.p; A.node_hi = TP.node_hi.p; A.leaf_sub_off = TP.leaf_sub_off.p;
+    A.slots = TP.slots.p; A.nslot = TP.nslot;
+    switch (P->kind) {
+      case HOR_SL_ON: launch_leaf<HOR_SL_ON>(A, mixed, TP.n_near_items, s); break;
+      case HOR_SL_OFF: launch_leaf<HOR_SL_OFF>(A, mixed, TP.n_near_items, s); break;
+      case HOR_DL_ON: launch_leaf<HOR_DL_ON>(A, mixed, TP.n_near_items, s); break;
+      default: launch_leaf<HOR_DL_OFF>(A, mixed, TP.n_near_items, s); break;
+    }
+    HOR_HIP_TRY(hipGetLastError());
+  }
+  // 3. far field
+  if (TP.n_far_items > 0) {
+    FarArgs F;
+    F.np1 = TP.np1; F.pflat = TP.pflat; F.ntp = P->ntp; F.nslot = TP.nslot;
+    F.item_node = TP.far_item_node.p; F.item_off = TP.far_item_off.p; F.item_cnt = TP.far_item_cnt.p;
+    F.tgt = TP.far_tgt.p; F.slot = TP.far_slot.p; F.far_idx = TP.far_idx.p;
+    F.cheb = TP.cheb.p; F.mom = TP.mom.p; F.tx = P->tx.p; F.tsub2 = P->tsub2.p; F.slots = TP.slots.p;
+    dim3 grid(TP.n_far_items);
+    if (dl) {
+      if (mixed) hipLaunchKernelGGL((k_far<true, true>), grid, dim3(kBlock), 0, s, F);
+      else hipLaunchKernelGGL((k_far<true, false>), grid, dim3(kBlock), 0, s, F);
+    } else {
+      if (mixed) hipLaunchKernelGGL((k_far<false, true>), grid, dim3(kBlock), 0, s, F);
+      else hipLaunchKernelGGL((k_far<false, false>), grid, dim3(kBlock), 0, s, F);
+    }
+    HOR_HIP_TRY(hipGetLastError());
+  }
+  HOR_HIP_TRY(hipEventRecord(c->ev[3], s));
+  // 4. per-target reduction
+  hipLaunchKernelGGL(k_tree_finalize, dim3((P->nt + kBlock - 1) / kBlock), dim3(kBlock), 0, s, P->nt, P->ntp,
+                     TP.nslot, P->tperm.p, TP.slot_start.p, TP.slots.p,
+                     dl ? -6.0 / (8.0 * kPi) : 1.0 / (8.0 * kPi),   // KITC.cpp:1395, 1612
+                     dl ? P->xi.p : nullptr, dl ? P->tsub2.p : nullptr, out_dev, base, coef, accumulate);
+  HOR_HIP_TRY(hipGetLastError());
+  return HOR_OK;
+}
+
+}  // namespace hor
+
+hor_plan::hor_plan() = default;
+hor_plan::~hor_plan() = default;

@@ -1,0 +1,60 @@
+// Operator plan: everything an evaluation needs that depends on geometry only.
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "device.h"
+#include "direct.h"
+
+namespace hor {
+struct TreePlan;  // kitc.hip
+}
+
+struct hor_plan {
+  hor_ctx* ctx = nullptr;
+  int kind = 0, method = 0, precision = 0;
+  double DEL = 0;
+  int ns = 0, nsp = 0, nt = 0, ntp = 0;
+  int ntile = 0, tiles_per_chunk = 0, nchunk = 0;
+  std::vector<int> sperm_h, tperm_h;
+  // sources, sorted SoA (stride nsp)
+  hor::DBuf<double> sx, sn, sa, sd, sub_box, sn_orig;
+  hor::DBuf<int> sperm;
+  // targets, sorted SoA (stride ntp)
+  hor::DBuf<int> tperm, tnear;
+  hor::DBuf<double> tx, tcoef, tsub, tsub2, tx0, tn0, tb, n0_orig, wave_box, xi, partial;
+  // staging for the host-pointer API
+  hor::DBuf<double> in_dens, in_d0, out_buf;
+  // statistics
+  double direct_pairs = 0, near_pairs = 0, proxy_pairs = 0, moment_terms = 0;
+  std::unique_ptr<hor::TreePlan> tree;
+  hor_plan();
+  ~hor_plan();
+};
+
+namespace hor {
+
+bool target_is_near(int kind, int method, double b, double DEL);
+void target_coefs(int kind, bool nearp, double b, double DEL, double q[kNumCoef]);
+int setup_targets(hor_plan* P, const double* tx, const double* tb, const double* tx0, const double* tn0,
+                  const double* sx_orig);
+int setup_sources(hor_plan* P, const std::vector<int>& perm, const double* sx, const double* sn,
+                  const double* sa);
+DirectArgs direct_args(hor_plan* P);
+double op_scale(int kind);
+int prepare_targets(hor_plan* P, const double* dens_dev, const double* d0_dev);
+
+// Evaluates into out_dev (caller order, 3*nt):
+//   accumulate=0, base=NULL: out = coef*op;  accumulate=0: out = base + coef*op;
+//   accumulate=1: out += coef*op.
+int plan_eval_into(hor_plan* P, const double* dens_dev, const double* d0_dev, double* out_dev,
+                   const double* base, double coef, int accumulate);
+
+int tree_plan_create(hor_plan* P, const double* sx, const double* sn, const double* sa,
+                     const double* tx, const double* tb, const double* tx0, const double* tn0,
+                     const double* ctr, const hor_tree_params& tp);
+int tree_plan_eval(hor_plan* P, const double* dens_dev, const double* d0_dev, double* out_dev,
+                   const double* base, double coef, int accumulate);
+
+}  // namespace hor

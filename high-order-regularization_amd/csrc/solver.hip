@@ -1,0 +1,469 @@
+// Two-surface second-kind solve on the device (highreg_stokes_2surf/main.cpp:140-677).
+//
+// The matvec A(u) = u - 2/3 [DL11 + DL21 ; DL12 + DL22](u) is four operator
+// plans evaluated straight into the output vector, after the density at each
+// target's nearest point is interpolated on its 6x6 Monge stencil.  The
+// stencil indices and Lagrange weights depend on geometry only, so they are
+// found once on the host (the reference re-scans every quadrature point for
+// every target on every matvec, main.cpp:969-1006).  GMRES keeps its Krylov
+// basis in HBM; only the Hessenberg column and the rotations live on the host.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <unordered_map>
+#include <vector>
+
+#include "plan.h"
+
+using namespace hor;
+
+namespace hor {
+
+// sixPoint_coeff (main.cpp:750-757): 6-point Lagrange weights at offset p
+// for nodes -2..3.
+static void six_coeff(double p, double a[6]) {
+  a[0] = -p * (p * p - 1.0) * (p - 2.0) * (p - 3.0) / 120.0;
+  a[1] = p * (p - 1.0) * (p * p - 4.0) * (p - 3.0) / 24.0;
+  a[2] = -(p * p - 1.0) * (p * p - 4.0) * (p - 3.0) / 12.0;
+  a[3] = p * (p + 1.0) * (p * p - 4.0) * (p - 3.0) / 12.0;
+  a[4] = -p * (p * p - 1.0) * (p + 2.0) * (p - 3.0) / 24.0;
+  a[5] = p * (p * p - 1.0) * (p * p - 4.0) / 120.0;
+}
+
+// Stencil of every target (interpolate_DL_density + sixPoint_stencil,
+// main.cpp:682-748, 916-1046): slot [k1][k2] holds the last source point (in
+// index order) whose in-plane coordinates equal the slot's grid node within
+// 1e-14 and that lies on the same side (sign of the normal coordinate) as the
+// target's nearest point; 0 when no point matches (the reference's "unset").
+// Candidates are found through a hash of rounded grid coordinates instead of
+// the reference's full scan; the acceptance tests are the reference's.
+static void build_stencils(const double ctr[3], double h, int ns, const double* sx, int nt,
+                           const int* nrst, const double* x0, const int* i123, std::vector<int>& st,
+                           std::vector<double>& co) {
+  static const int pairs[3][2] = {{1, 2}, {0, 2}, {0, 1}};  // (i1, i2) for i3 = 0, 1, 2
+  std::unordered_map<uint64_t, std::vector<int>> bucket[3];
+  auto key = [](long l1, long l2) { return ((uint64_t)(uint32_t)(int32_t)l1 << 32) | (uint32_t)(int32_t)l2; };
+  for (int o = 0; o < 3; o++) {
+    int a = pairs[o][0], b = pairs[o][1];
+    for (int j = 0; j < ns; j++) {
+      long l1 = std::lround((sx[3 * j + a] - ctr[a]) / h);
+      long l2 = std::lround((sx[3 * j + b] - ctr[b]) / h);
+      bucket[o][key(l1, l2)].push_back(j);
+    }
+  }
+  st.assign(36 * (size_t)nt, 0);
+  co.assign(12 * (size_t)nt, 0.0);
+  const double tol = 1.e-14;
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int i = 0; i < nt; i++) {
+    if (nrst[i] == 0) continue;
+    int ind1 = i123[3 * i], ind2 = i123[3 * i + 1], ind3 = i123[3 * i + 2];
+    const double* p0 = x0 + 3 * i;
+    double xLow = std::floor((p0[ind1] - ctr[ind1]) / h) * h + ctr[ind1];
+    double yLow = std::floor((p0[ind2] - ctr[ind2]) / h) * h + ctr[ind2];
+    double xHigh = xLow + h, yHigh = yLow + h;
+    double z0 = p0[ind3] - ctr[ind3];
+    double cx[6] = {xLow - 2.0 * h, xLow - h, xLow, xHigh, xHigh + h, xHigh + 2.0 * h};
+    double cy[6] = {yLow - 2.0 * h, yLow - h, yLow, yHigh, yHigh + h, yHigh + 2.0 * h};
+    // orientation whose bucket coordinates are (ind1, ind2)
+    int o = (ind3 == 0) ? 0 : (ind3 == 1) ? 1 : 2;
+    bool swapped = (pairs[o][0] != ind1);  // i123 always lists ind1 < ind2 for the reference patches
+    for (int c = 0; c < 6; c++)
+      for (int r = 0; r < 6; r++) {
+        long l1 = std::lround((cx[c] - ctr[ind1]) / h);
+        long l2 = std::lround((cy[r] - ctr[ind2]) / h);
+        auto it = bucket[o].find(swapped ? key(l2, l1) : key(l1, l2));
+        if (it == bucket[o].end()) continue;
+        int best = 0;
+        for (int j : it->second) {
+          double z = sx[3 * j + ind3] - ctr[ind3];
+          if (!(((z0 > 0) && (z > 0)) || ((z0 < 0) && (z < 0)))) continue;
+          double X = sx[3 * j + ind1], Y = sx[3 * j + ind2];
+          int col = -1, row = -1;
+          for (int q = 0; q < 6; q++)
+            if (std::fabs(cx[q] - X) < tol) { col = q; break; }
+          if (col != c) continue;
+          for (int q = 0; q < 6; q++)
+            if (std::fabs(cy[q] - Y) < tol) { row = q; break; }
+          if (row != r) continue;
+          best = std::max(best, j);
+        }
+        st[36 * (size_t)i + 6 * c + r] = best;
+      }
+    double p = (p0[ind1] - xLow) / h;
+    double q = (p0[ind2] - yLow) / h;
+    six_coeff(p, &co[12 * (size_t)i]);
+    six_coeff(q, &co[12 * (size_t)i + 6]);
+  }
+}
+
+// g0 = sum_j b_j sum_i a_i g[st[i][j]] (sixPointInterpolate, main.cpp:759-778).
+__global__ void k_interp(int nt, const int* nrst, const int* st, const double* co, const double* g,
+                         double* g0) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nt) return;
+  if (nrst[i] == 0) {
+    g0[3 * i] = g0[3 * i + 1] = g0[3 * i + 2] = 0.0;
+    return;
+  }
+  const int* s = st + 36 * (size_t)i;
+  const double* a = co + 12 * (size_t)i;
+  const double* b = a + 6;
+  for (int c = 0; c < 3; c++) {
+    double v = 0.0;
+    for (int j = 0; j < 6; j++) {
+      double f = 0.0;
+      for (int k = 0; k < 6; k++) f += a[k] * g[3 * s[6 * k + j] + c];
+      v += b[j] * f;
+    }
+    g0[3 * i + c] = v;
+  }
+}
+
+// ---- vector kernels for GMRES
+constexpr int kRedBlocks = 1024;
+
+__global__ void k_dot_partial(int n, const double* x, const double* y, double* part) {
+  __shared__ double sm[kBlock];
+  double s = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) s += x[i] * y[i];
+  sm[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) sm[threadIdx.x] += sm[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
+}
+
+__global__ void k_axpy(int n, double a, const double* x, double* y) {  // y += a x
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] += a * x[i];
+}
+
+__global__ void k_scale(int n, double a, const double* x, double* y) {  // y = a x
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = a * x[i];
+}
+
+__global__ void k_sub(int n, const double* a, const double* b, double* y) {  // y = a - b
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = a[i] - b[i];
+}
+
+}  // namespace hor
+
+struct hor_solver {
+  hor_ctx* ctx = nullptr;
+  double h = 0, DEL = 0;
+  int method = 0, precision = 0;
+  int n1 = 0, n2 = 0, n = 0;
+  hor_tree_params tp[2];
+  // host copies needed for lazily built SL plans
+  std::vector<double> x[2], nrml[2], area[2], x0[2], n0[2], b[2];
+  double ctr[2][3];
+  hor_plan* on[2] = {nullptr, nullptr};
+  hor_plan* off[2] = {nullptr, nullptr};   // off[s]: sources of surface s at the other surface's points
+  hor_plan* sl_on[2] = {nullptr, nullptr};
+  hor_plan* sl_off[2] = {nullptr, nullptr};
+  DBuf<int> nrst[2], st[2];
+  DBuf<double> co[2], g0[2];
+  DBuf<double> vin, vout, V, w, part, bvec, xvec, r;
+  std::vector<double> part_h;
+  ~hor_solver() {
+    for (int k = 0; k < 2; k++) {
+      hor_plan_destroy(on[k]);
+      hor_plan_destroy(off[k]);
+      hor_plan_destroy(sl_on[k]);
+      hor_plan_destroy(sl_off[k]);
+    }
+  }
+};
+
+static int dot_dev(hor_solver* S, const double* x, const double* y, double* out) {
+  hipStream_t s = S->ctx->stream;
+  hipLaunchKernelGGL(k_dot_partial, dim3(kRedBlocks), dim3(kBlock), 0, s, S->n, x, y, S->part.p);
+  HOR_HIP_TRY(hipGetLastError());
+  HOR_HIP_TRY(hipMemcpyAsync(S->part_h.data(), S->part.p, sizeof(double) * kRedBlocks, hipMemcpyDeviceToHost, s));
+  HOR_HIP_TRY(hipStreamSynchronize(s));
+  double v = 0.0;
+  for (int k = 0; k < kRedBlocks; k++) v += S->part_h[k];
+  *out = v;
+  return HOR_OK;
+}
+
+static inline dim3 grid_for(int n) { return dim3((n + kBlock - 1) / kBlock); }
+
+// A(u) on device pointers (main.cpp:541-677).
+static int matvec_dev(hor_solver* S, const double* u, double* Au) {
+  hipStream_t s = S->ctx->stream;
+  const double factor = 2.0 / 3.0;
+  const double* u1 = u;
+  const double* u2 = u + 3 * (size_t)S->n1;
+  double* A1 = Au;
+  double* A2 = Au + 3 * (size_t)S->n1;
+  // g0 at the points of surface 2 from the density of surface 1, and vice versa
+  hipLaunchKernelGGL(k_interp, grid_for(S->n2), dim3(kBlock), 0, s, S->n2, S->nrst[1].p, S->st[1].p,
+                     S->co[1].p, u1, S->g0[1].p);
+  hipLaunchKernelGGL(k_interp, grid_for(S->n1), dim3(kBlock), 0, s, S->n1, S->nrst[0].p, S->st[0].p,
+                     S->co[0].p, u2, S->g0[0].p);
+  HOR_HIP_TRY(hipGetLastError());
+  HOR_TRY(plan_eval_into(S->on[0], u1, nullptr, A1, u1, -factor, 0));        // u1 - 2/3 DL11
+  HOR_TRY(plan_eval_into(S->off[0], u1, S->g0[1].p, A2, u2, -factor, 0));    // u2 - 2/3 DL12
+  HOR_TRY(plan_eval_into(S->off[1], u2, S->g0[0].p, A1, nullptr, -factor, 1));  // - 2/3 DL21
+  HOR_TRY(plan_eval_into(S->on[1], u2, nullptr, A2, nullptr, -factor, 1));   // - 2/3 DL22
+  return HOR_OK;
+}
+
+extern "C" int hor_solver_create(hor_ctx* ctx, double h, double DEL, int method, int precision,
+                                 const hor_tree_params* tree1, const hor_tree_params* tree2,
+                                 const hor_surface_desc* s1, const hor_surface_desc* s2,
+                                 hor_solver** out) {
+  if (!ctx || !s1 || !s2 || !out) return set_error(HOR_E_ARG, "hor_solver_create: null argument");
+  *out = nullptr;
+  if (!(h > 0) || !(DEL > 0)) return set_error(HOR_E_ARG, "hor_solver_create: h and DEL must be > 0");
+  const hor_surface_desc* sd[2] = {s1, s2};
+  for (int k = 0; k < 2; k++)
+    if (sd[k]->n <= 0 || !sd[k]->x || !sd[k]->nrml || !sd[k]->area || !sd[k]->nrst || !sd[k]->x0 ||
+        !sd[k]->n0 || !sd[k]->b || !sd[k]->i123)
+      return set_error(HOR_E_ARG, "hor_solver_create: surface %d incomplete", k + 1);
+  HOR_HIP_TRY(hipSetDevice(ctx->device));
+  std::unique_ptr<hor_solver> S(new hor_solver());
+  S->ctx = ctx;
+  S->h = h;
+  S->DEL = DEL;
+  S->method = method;
+  S->precision = precision;
+  S->n1 = s1->n;
+  S->n2 = s2->n;
+  S->n = 3 * (S->n1 + S->n2);
+  S->tp[0] = tree1 ? *tree1 : hor_tree_params{6, 1000, 0.6};
+  S->tp[1] = tree2 ? *tree2 : hor_tree_params{6, 1000, 0.6};
+  hipStream_t st = ctx->stream;
+  for (int k = 0; k < 2; k++) {
+    const hor_surface_desc* d = sd[k];
+    int n = d->n;
+    S->x[k].assign(d->x, d->x + 3 * n);
+    S->nrml[k].assign(d->nrml, d->nrml + 3 * n);
+    S->area[k].assign(d->area, d->area + n);
+    S->x0[k].assign(d->x0, d->x0 + 3 * n);
+    S->n0[k].assign(d->n0, d->n0 + 3 * n);
+    S->b[k].assign(d->b, d->b + n);
+    for (int c = 0; c < 3; c++) S->ctr[k][c] = d->ctr[c];
+  }
+  for (int k = 0; k < 2; k++) {
+    int o = 1 - k;
+    const hor_surface_desc* me = sd[k];
+    const hor_surface_desc* other = sd[o];
+    // on-surface DL of surface k
+    HOR_TRY(hor_plan_create(ctx, HOR_DL_ON, method, precision, DEL, me->n, me->x, me->nrml, me->area, me->n,
+                            me->x, nullptr, nullptr, nullptr, me->ctr, &S->tp[k], &S->on[k]));
+    // DL of surface k at the points of the other surface (whose nearest data is relative to k)
+    HOR_TRY(hor_plan_create(ctx, HOR_DL_OFF, method, precision, DEL, me->n, me->x, me->nrml, me->area,
+                            other->n, other->x, other->b, other->x0, other->n0, me->ctr, &S->tp[k],
+                            &S->off[k]));
+    // stencils of the points of surface k (their nearest points lie on the other surface)
+    std::vector<int> stv;
+    std::vector<double> cov;
+    build_stencils(other->ctr, h, other->n, other->x, me->n, me->nrst, me->x0, me->i123, stv, cov);
+    HOR_TRY(S->st[k].upload(stv, st));
+    HOR_TRY(S->co[k].upload(cov, st));
+    HOR_TRY(S->nrst[k].upload(me->nrst, me->n, st));
+    HOR_TRY(S->g0[k].alloc(3 * (size_t)me->n));
+  }
+  HOR_TRY(S->vin.alloc(S->n));
+  HOR_TRY(S->vout.alloc(S->n));
+  HOR_TRY(S->part.alloc(kRedBlocks));
+  S->part_h.assign(kRedBlocks, 0.0);
+  HOR_HIP_TRY(hipStreamSynchronize(st));
+  *out = S.release();
+  return HOR_OK;
+}
+
+extern "C" int hor_solver_destroy(hor_solver* S) {
+  if (!S) return HOR_OK;
+  (void)hipSetDevice(S->ctx->device);
+  (void)hipStreamSynchronize(S->ctx->stream);
+  delete S;
+  return HOR_OK;
+}
+
+extern "C" int hor_solver_size(hor_solver* S, int* n) {
+  if (!S || !n) return set_error(HOR_E_ARG, "null");
+  *n = S->n;
+  return HOR_OK;
+}
+
+extern "C" int hor_solver_matvec_dev(hor_solver* S, const double* u_dev, double* Au_dev) {
+  if (!S || !u_dev || !Au_dev) return set_error(HOR_E_ARG, "hor_solver_matvec_dev: null");
+  hor_ctx* c = S->ctx;
+  HOR_HIP_TRY(hipSetDevice(c->device));
+  HOR_HIP_TRY(hipEventRecord(c->ev[0], c->stream));
+  HOR_TRY(matvec_dev(S, u_dev, Au_dev));
+  HOR_HIP_TRY(hipEventRecord(c->ev[1], c->stream));
+  HOR_HIP_TRY(hipStreamSynchronize(c->stream));
+  float t = 0;
+  HOR_HIP_TRY(hipEventElapsedTime(&t, c->ev[0], c->ev[1]));
+  c->last_total_ms = t;
+  return HOR_OK;
+}
+
+extern "C" int hor_solver_matvec(hor_solver* S, const double* u, double* Au) {
+  if (!S || !u || !Au) return set_error(HOR_E_ARG, "hor_solver_matvec: null");
+  hipStream_t s = S->ctx->stream;
+  HOR_HIP_TRY(hipSetDevice(S->ctx->device));
+  HOR_HIP_TRY(hipMemcpyAsync(S->vin.p, u, sizeof(double) * S->n, hipMemcpyHostToDevice, s));
+  HOR_TRY(hor_solver_matvec_dev(S, S->vin.p, S->vout.p));
+  HOR_HIP_TRY(hipMemcpyAsync(Au, S->vout.p, sizeof(double) * S->n, hipMemcpyDeviceToHost, s));
+  HOR_HIP_TRY(hipStreamSynchronize(s));
+  return HOR_OK;
+}
+
+extern "C" int hor_solver_interp_g0(hor_solver* S, int k, const double* u, double* g0) {
+  if (!S || !u || !g0 || (k != 1 && k != 2)) return set_error(HOR_E_ARG, "hor_solver_interp_g0: bad argument");
+  hipStream_t s = S->ctx->stream;
+  HOR_HIP_TRY(hipSetDevice(S->ctx->device));
+  int me = k - 1, other = 1 - me;
+  int nother = other == 0 ? S->n1 : S->n2, nme = me == 0 ? S->n1 : S->n2;
+  HOR_HIP_TRY(hipMemcpyAsync(S->vin.p, u, sizeof(double) * 3 * nother, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_interp, grid_for(nme), dim3(kBlock), 0, s, nme, S->nrst[me].p, S->st[me].p, S->co[me].p,
+                     S->vin.p, S->g0[me].p);
+  HOR_HIP_TRY(hipGetLastError());
+  HOR_HIP_TRY(hipMemcpyAsync(g0, S->g0[me].p, sizeof(double) * 3 * nme, hipMemcpyDeviceToHost, s));
+  HOR_HIP_TRY(hipStreamSynchronize(s));
+  return HOR_OK;
+}
+
+// eval_SL (main.cpp:421-537).
+extern "C" int hor_solver_eval_sl(hor_solver* S, const double* f1, const double* f2, const double* f01,
+                                  const double* f02, double* sl) {
+  if (!S || !f1 || !f2 || !f01 || !f02 || !sl) return set_error(HOR_E_ARG, "hor_solver_eval_sl: null");
+  hor_ctx* c = S->ctx;
+  HOR_HIP_TRY(hipSetDevice(c->device));
+  for (int k = 0; k < 2; k++) {
+    int o = 1 - k;
+    int nk = (int)S->area[k].size(), no = (int)S->area[o].size();
+    if (!S->sl_on[k])
+      HOR_TRY(hor_plan_create(c, HOR_SL_ON, S->method, S->precision, S->DEL, nk, S->x[k].data(),
+                              S->nrml[k].data(), S->area[k].data(), nk, S->x[k].data(), nullptr, nullptr,
+                              nullptr, S->ctr[k], &S->tp[k], &S->sl_on[k]));
+    if (!S->sl_off[k])
+      HOR_TRY(hor_plan_create(c, HOR_SL_OFF, S->method, S->precision, S->DEL, nk, S->x[k].data(),
+                              S->nrml[k].data(), S->area[k].data(), no, S->x[o].data(), S->b[o].data(),
+                              S->x0[o].data(), S->n0[o].data(), S->ctr[k], &S->tp[k], &S->sl_off[k]));
+  }
+  hipStream_t s = c->stream;
+  int n1 = S->n1, n2 = S->n2;
+  DBuf<double> F, F0;
+  HOR_TRY(F.alloc(3 * (size_t)(n1 + n2)));
+  HOR_TRY(F0.alloc(3 * (size_t)(n1 + n2)));
+  HOR_HIP_TRY(hipMemcpyAsync(F.p, f1, sizeof(double) * 3 * n1, hipMemcpyHostToDevice, s));
+  HOR_HIP_TRY(hipMemcpyAsync(F.p + 3 * n1, f2, sizeof(double) * 3 * n2, hipMemcpyHostToDevice, s));
+  HOR_HIP_TRY(hipMemcpyAsync(F0.p, f01, sizeof(double) * 3 * n1, hipMemcpyHostToDevice, s));
+  HOR_HIP_TRY(hipMemcpyAsync(F0.p + 3 * n1, f02, sizeof(double) * 3 * n2, hipMemcpyHostToDevice, s));
+  double* o1 = S->vout.p;
+  double* o2 = S->vout.p + 3 * n1;
+  HOR_TRY(plan_eval_into(S->sl_on[0], F.p, nullptr, o1, nullptr, 1.0, 0));            // SL11
+  HOR_TRY(plan_eval_into(S->sl_off[0], F.p, F0.p + 3 * n1, o2, nullptr, 1.0, 0));     // SL12
+  HOR_TRY(plan_eval_into(S->sl_off[1], F.p + 3 * n1, F0.p, o1, nullptr, 1.0, 1));     // + SL21
+  HOR_TRY(plan_eval_into(S->sl_on[1], F.p + 3 * n1, nullptr, o2, nullptr, 1.0, 1));   // + SL22
+  HOR_HIP_TRY(hipMemcpyAsync(sl, S->vout.p, sizeof(double) * S->n, hipMemcpyDeviceToHost, s));
+  HOR_HIP_TRY(hipStreamSynchronize(s));
+  return HOR_OK;
+}
+
+// Restarted GMRES(m) with MGS Arnoldi and Givens rotations (main.cpp:210-416);
+// restarts from the true residual b - A x.
+extern "C" int hor_solver_gmres(hor_solver* S, const double* b, double* x, int m, double tol, int maxit,
+                                int* iters, double* relres, double* hist) {
+  if (!S || !b || !x || m < 1 || maxit < 0) return set_error(HOR_E_ARG, "hor_solver_gmres: bad argument");
+  hor_ctx* c = S->ctx;
+  hipStream_t s = c->stream;
+  HOR_HIP_TRY(hipSetDevice(c->device));
+  const int n = S->n;
+  HOR_TRY(S->V.alloc((size_t)n * (m + 1)));
+  HOR_TRY(S->w.alloc(n));
+  HOR_TRY(S->bvec.alloc(n));
+  HOR_TRY(S->xvec.alloc(n));
+  HOR_TRY(S->r.alloc(n));
+  HOR_HIP_TRY(hipMemcpyAsync(S->bvec.p, b, sizeof(double) * n, hipMemcpyHostToDevice, s));
+  HOR_HIP_TRY(hipMemsetAsync(S->xvec.p, 0, sizeof(double) * n, s));
+  std::vector<double> H((size_t)(m + 1) * m, 0.0), g(m + 1, 0.0), y(m + 1, 0.0), cs(m, 0.0), sn(m, 0.0);
+  auto Hh = [&](int i, int j) -> double& { return H[(size_t)i * m + j]; };
+  auto Vp = [&](int i) { return S->V.p + (size_t)i * n; };
+  dim3 gr = grid_for(n), bl(kBlock);
+  double bnorm2;
+  HOR_TRY(dot_dev(S, S->bvec.p, S->bvec.p, &bnorm2));
+  double b_norm = std::sqrt(bnorm2);
+  int iter = 0;
+  double error = 1.0;
+  while (true) {
+    if (error <= tol || iter >= maxit) break;
+    // r = b - A x (x = 0 on the first cycle: r = b)
+    if (iter > 0) {
+      HOR_TRY(matvec_dev(S, S->xvec.p, S->r.p));
+      hipLaunchKernelGGL(k_sub, gr, bl, 0, s, n, S->bvec.p, S->r.p, Vp(0));
+    } else {
+      HOR_HIP_TRY(hipMemcpyAsync(Vp(0), S->bvec.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    }
+    double rn2;
+    HOR_TRY(dot_dev(S, Vp(0), Vp(0), &rn2));
+    double r_norm = std::sqrt(rn2);
+    error = r_norm;
+    if (b_norm > 1.e-8) error /= b_norm;
+    if (error <= tol || r_norm == 0.0) break;
+    hipLaunchKernelGGL(k_scale, gr, bl, 0, s, n, 1.0 / r_norm, Vp(0), Vp(0));
+    std::fill(g.begin(), g.end(), 0.0);
+    g[0] = r_norm;
+    int j = -1;
+    while ((j < m - 1) && (error > tol) && (iter < maxit)) {
+      j = j + 1;
+      iter = iter + 1;
+      HOR_TRY(matvec_dev(S, Vp(j), Vp(j + 1)));
+      for (int i = 0; i <= j; i++) {  // modified Gram-Schmidt
+        double t;
+        HOR_TRY(dot_dev(S, Vp(i), Vp(j + 1), &t));
+        Hh(i, j) = t;
+        hipLaunchKernelGGL(k_axpy, gr, bl, 0, s, n, -t, Vp(i), Vp(j + 1));
+      }
+      double vn2;
+      HOR_TRY(dot_dev(S, Vp(j + 1), Vp(j + 1), &vn2));
+      double vn = std::sqrt(vn2);
+      Hh(j + 1, j) = vn;
+      if (vn != 0.0) hipLaunchKernelGGL(k_scale, gr, bl, 0, s, n, 1.0 / vn, Vp(j + 1), Vp(j + 1));
+      for (int i = 0; i <= j - 1; i++) {
+        double t = Hh(i, j);
+        Hh(i, j) = cs[i] * t + sn[i] * Hh(i + 1, j);
+        Hh(i + 1, j) = -sn[i] * t + cs[i] * Hh(i + 1, j);
+      }
+      double hh = std::sqrt(Hh(j, j) * Hh(j, j) + Hh(j + 1, j) * Hh(j + 1, j));
+      if (hh == 0.0) hh = 1.e-16;
+      cs[j] = Hh(j, j) / hh;
+      sn[j] = Hh(j + 1, j) / hh;
+      Hh(j, j) = cs[j] * Hh(j, j) + sn[j] * Hh(j + 1, j);
+      Hh(j + 1, j) = 0.0;
+      g[j + 1] = -sn[j] * g[j];
+      g[j] = cs[j] * g[j];
+      error = std::fabs(g[j + 1]);
+      if (b_norm > 1.e-8) error /= b_norm;
+      if (hist) hist[iter - 1] = error;
+    }
+    // x += V y with H y = g (upper triangular)
+    for (int i = 0; i <= j; i++) y[i] = g[i];
+    y[j] = y[j] / Hh(j, j);
+    for (int k = j - 1; k >= 0; k--) {
+      double t = y[k];
+      for (int i = k + 1; i <= j; i++) t -= Hh(k, i) * y[i];
+      y[k] = t / Hh(k, k);
+    }
+    for (int i = 0; i <= j; i++) hipLaunchKernelGGL(k_axpy, gr, bl, 0, s, n, y[i], Vp(i), S->xvec.p);
+    HOR_HIP_TRY(hipGetLastError());
+  }
+  HOR_HIP_TRY(hipMemcpyAsync(x, S->xvec.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+  HOR_HIP_TRY(hipStreamSynchronize(s));
+  if (iters) *iters = iter;
+  if (relres) *relres = error;
+  return HOR_OK;
+}

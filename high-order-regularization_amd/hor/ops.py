@@ -1,0 +1,182 @@
+"""Operator API over libhor.so: context, plans, and the reference's functions.
+
+The module-level functions carry the reference's names and argument meaning
+(stokeslets.h:9-29, stresslets.h:9-30, KITC.h:99-118) with surfaces/targets as
+dicts (see hor.geometry).  Everything runs on the GPU; a missing library or
+device raises instead of falling back to the CPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import DIRECT, DL_OFF, DL_ON, FP64, MIXED, SL_OFF, SL_ON, TREE, check, f64, ptr
+
+__all__ = ["Context", "Plan", "TreeParams", "SL_ON", "SL_OFF", "DL_ON", "DL_OFF", "DIRECT", "TREE",
+           "FP64", "MIXED", "eval_Stokes_SL_offSurf", "eval_Stokes_SL_onSurf",
+           "eval_Stokes_DL_offSurf", "eval_Stokes_DL_onSurf", "eval_Stokes_SL_onSurf_tree",
+           "eval_Stokes_SL_offSurf_tree", "eval_Stokes_DL_onSurf_tree", "eval_Stokes_DL_offSurf_tree",
+           "device_count"]
+
+
+def device_count():
+    n = C.c_int()
+    check(_lib.lib().hor_device_count(C.byref(n)), "hor_device_count")
+    return n.value
+
+
+def TreeParams(P=6, N0=1000, theta=0.6):
+    """Treecode parameters (KITC.h:14-17 defaults)."""
+    t = _lib.TreeParams()
+    t.P, t.N0, t.theta = int(P), int(N0), float(theta)
+    return t
+
+
+class Context:
+    """One GPU, one HIP stream."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        check(_lib.lib().hor_ctx_create(int(device), C.byref(h)), "hor_ctx_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            _lib.lib().hor_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def last_timing(self):
+        """(total ms, dominant-kernel ms) of the last evaluation (HIP events)."""
+        a, b = C.c_double(), C.c_double()
+        check(_lib.lib().hor_ctx_last_timing(self.h, C.byref(a), C.byref(b)), "hor_ctx_last_timing")
+        return a.value, b.value
+
+    # device memory helpers
+    def alloc(self, nbytes):
+        p = C.c_void_p()
+        check(_lib.lib().hor_dev_alloc(self.h, int(nbytes), C.byref(p)), "hor_dev_alloc")
+        return p
+
+    def free(self, p):
+        check(_lib.lib().hor_dev_free(self.h, p), "hor_dev_free")
+
+    def upload(self, dst, arr):
+        arr = np.ascontiguousarray(arr)
+        check(_lib.lib().hor_dev_upload(self.h, dst, ptr(arr), arr.nbytes), "hor_dev_upload")
+
+    def download(self, src, out):
+        check(_lib.lib().hor_dev_download(self.h, ptr(out), src, out.nbytes), "hor_dev_download")
+        return out
+
+
+class Plan:
+    """Geometry-bound operator (hor_plan_create)."""
+
+    def __init__(self, ctx, kind, src, tgt=None, DEL=None, method=DIRECT, precision=FP64,
+                 ctr=None, tree=None):
+        ns = src["n"]
+        on = kind in (SL_ON, DL_ON)
+        if on:
+            nt = ns if tgt is None else tgt["n"]
+            tx = src["x"] if tgt is None else tgt["x"]
+            tb = tx0 = tn0 = None
+        else:
+            nt = tgt["n"]
+            tx, tb, tx0, tn0 = tgt["x"], tgt["b"], tgt.get("x0"), tgt["n0"]
+        self._keep = [f64(src["x"], 3 * ns), f64(src["nrml"], 3 * ns), f64(src["area"], ns),
+                      f64(tx, 3 * nt), f64(tb), f64(tx0), f64(tn0)]
+        k = self._keep
+        c = None
+        if ctr is not None:
+            c = f64(np.asarray(ctr), 3)
+            self._keep.append(c)
+        tp = None if tree is None else C.byref(tree)
+        h = C.c_void_p()
+        check(_lib.lib().hor_plan_create(ctx.h, int(kind), int(method), int(precision), float(DEL), ns,
+                                         ptr(k[0]), ptr(k[1]), ptr(k[2]), nt, ptr(k[3]), ptr(k[4]),
+                                         ptr(k[5]), ptr(k[6]), ptr(c), tp, C.byref(h)), "hor_plan_create")
+        self.h, self.ctx, self.kind, self.ns, self.nt = h, ctx, kind, ns, nt
+
+    def close(self):
+        if self.h:
+            _lib.lib().hor_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __call__(self, dens, d0=None):
+        dens = f64(dens, 3 * self.ns)
+        d0 = None if d0 is None else f64(d0, 3 * self.nt)
+        out = np.zeros(3 * self.nt)
+        check(_lib.lib().hor_plan_eval(self.h, ptr(dens), ptr(d0), ptr(out)), "hor_plan_eval")
+        return out
+
+    def eval_dev(self, dens_dev, d0_dev, out_dev):
+        check(_lib.lib().hor_plan_eval_dev(self.h, dens_dev, d0_dev, out_dev), "hor_plan_eval_dev")
+
+    def stats(self):
+        v = [C.c_double() for _ in range(4)]
+        check(_lib.lib().hor_plan_stats(self.h, *[C.byref(x) for x in v]), "hor_plan_stats")
+        return dict(direct_pairs=v[0].value, near_pairs=v[1].value, proxy_pairs=v[2].value,
+                    moment_terms=v[3].value)
+
+
+# ---------------------------------------------------------------------------
+# The reference's entry points (one-shot: plan, evaluate, release).
+
+def eval_Stokes_SL_offSurf(ctx, h, DEL, surf, targets):
+    """stokeslets.cpp:12-86; density surf['f'], nearest force targets['f0']."""
+    return Plan(ctx, SL_OFF, surf, targets, DEL)(surf["f"], targets["f0"])
+
+
+def eval_Stokes_SL_onSurf(ctx, DEL, surf, targets=None):
+    """stokeslets.cpp:141-180; target i is quadrature point i."""
+    return Plan(ctx, SL_ON, surf, targets, DEL)(surf["f"])
+
+
+def eval_Stokes_DL_offSurf(ctx, h, DEL, surf, targets):
+    """stresslets.cpp:13-96; density surf['g'], nearest density targets['g0']."""
+    return Plan(ctx, DL_OFF, surf, targets, DEL)(surf["g"], targets["g0"])
+
+
+def eval_Stokes_DL_onSurf(ctx, DEL, surf, targets=None):
+    """stresslets.cpp:186-228."""
+    return Plan(ctx, DL_ON, surf, targets, DEL)(surf["g"])
+
+
+def eval_Stokes_SL_onSurf_tree(ctx, ctr, h, DEL, surf, targets=None, tree=None):
+    """KITC.cpp:1389-1492; ctr = centre of the source surface's root box."""
+    return Plan(ctx, SL_ON, surf, targets, DEL, TREE, ctr=ctr, tree=tree or TreeParams())(surf["f"])
+
+
+def eval_Stokes_SL_offSurf_tree(ctx, ctr, h, DEL, surf, targets, tree=None):
+    """KITC.cpp:1496-1602."""
+    return Plan(ctx, SL_OFF, surf, targets, DEL, TREE, ctr=ctr, tree=tree or TreeParams())(surf["f"], targets["f0"])
+
+
+def eval_Stokes_DL_onSurf_tree(ctx, ctr, h, DEL, surf, targets=None, tree=None):
+    """KITC.cpp:1606-1709."""
+    return Plan(ctx, DL_ON, surf, targets, DEL, TREE, ctr=ctr, tree=tree or TreeParams())(surf["g"])
+
+
+def eval_Stokes_DL_offSurf_tree(ctx, ctr, h, DEL, surf, targets, tree=None):
+    """KITC.cpp:1712-1830."""
+    return Plan(ctx, DL_OFF, surf, targets, DEL, TREE, ctr=ctr, tree=tree or TreeParams())(surf["g"], targets["g0"])

@@ -1,0 +1,92 @@
+"""Two-surface second-kind solve (highreg_stokes_2surf/main.cpp:140-677) on the GPU.
+
+``Solver`` binds two surfaces (dicts with x, nrml, area, ctr and the nearest
+data of each surface's points relative to the other surface: nrst, x0, n0, b,
+i123) and provides the matvec (solve_IE_1step), the right-hand side
+(eval_SL), the stencil interpolation (interpolate_DL_density) and restarted
+GMRES with every vector resident in HBM.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import DIRECT, FP64, check, f64, i32, ptr
+
+
+def _desc(s, keep):
+    d = _lib.SurfaceDesc()
+    n = int(s["n"])
+    d.n = n
+    arrs = dict(x=f64(s["x"], 3 * n), nrml=f64(s["nrml"], 3 * n), area=f64(s["area"], n),
+                nrst=i32(s["nrst"], n), x0=f64(s["x0"], 3 * n), n0=f64(s["n0"], 3 * n),
+                b=f64(s["b"], n), i123=i32(s["i123"], 3 * n))
+    keep.extend(arrs.values())
+    for k, v in arrs.items():
+        setattr(d, k, ptr(v))
+    for k in range(3):
+        d.ctr[k] = float(s["ctr"][k])
+    return d
+
+
+class Solver:
+    def __init__(self, ctx, h, DEL, s1, s2, method=DIRECT, precision=FP64, tree1=None, tree2=None):
+        self._keep = []
+        self.d1, self.d2 = _desc(s1, self._keep), _desc(s2, self._keep)
+        t1 = None if tree1 is None else C.byref(tree1)
+        t2 = None if tree2 is None else C.byref(tree2)
+        h_ = C.c_void_p()
+        check(_lib.lib().hor_solver_create(ctx.h, float(h), float(DEL), int(method), int(precision), t1, t2,
+                                           C.byref(self.d1), C.byref(self.d2), C.byref(h_)),
+              "hor_solver_create")
+        self.h, self.ctx = h_, ctx
+        n = C.c_int()
+        check(_lib.lib().hor_solver_size(self.h, C.byref(n)), "hor_solver_size")
+        self.n = n.value
+        self.n1, self.n2 = int(s1["n"]), int(s2["n"])
+
+    def close(self):
+        if self.h:
+            _lib.lib().hor_solver_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def matvec(self, u):
+        u = f64(u, self.n)
+        out = np.zeros(self.n)
+        check(_lib.lib().hor_solver_matvec(self.h, ptr(u), ptr(out)), "hor_solver_matvec")
+        return out
+
+    def matvec_dev(self, u_dev, out_dev):
+        check(_lib.lib().hor_solver_matvec_dev(self.h, u_dev, out_dev), "hor_solver_matvec_dev")
+
+    def interp_g0(self, k, u_other):
+        """g0 at the points of surface k (1 or 2) from the other surface's density."""
+        n_other = self.n2 if k == 1 else self.n1
+        n_me = self.n1 if k == 1 else self.n2
+        u = f64(u_other, 3 * n_other)
+        out = np.zeros(3 * n_me)
+        check(_lib.lib().hor_solver_interp_g0(self.h, int(k), ptr(u), ptr(out)), "hor_solver_interp_g0")
+        return out
+
+    def eval_sl(self, f1, f2, f01, f02):
+        out = np.zeros(self.n)
+        a = [f64(f1, 3 * self.n1), f64(f2, 3 * self.n2), f64(f01, 3 * self.n1), f64(f02, 3 * self.n2)]
+        check(_lib.lib().hor_solver_eval_sl(self.h, *[ptr(x) for x in a], ptr(out)), "hor_solver_eval_sl")
+        return out
+
+    def gmres(self, b, m=15, tol=1e-10, maxit=100):
+        b = f64(b, self.n)
+        x = np.zeros(self.n)
+        hist = np.zeros(max(maxit, 1))
+        it, res = C.c_int(), C.c_double()
+        check(_lib.lib().hor_solver_gmres(self.h, ptr(b), ptr(x), int(m), float(tol), int(maxit), C.byref(it),
+                                          C.byref(res), ptr(hist)), "hor_solver_gmres")
+        return x, it.value, hist[: it.value], res.value

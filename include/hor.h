@@ -1,0 +1,287 @@
+/* hor.h - C ABI of the MI355X regularized-Stokes library (libhor.so).
+ *
+ * The library evaluates the high-order regularized Stokeslet (single layer,
+ * SL) and stresslet (double layer, DL) sums of Beale & Tlupova's method on one
+ * AMD Instinct MI355X (gfx950), directly or with the barycentric-Lagrange
+ * treecode (KITC), and drives the two-surface second-kind solve (matvec and
+ * GMRES) with every vector resident in HBM.
+ *
+ * Conventions
+ *   - All arrays are plain host pointers unless a name ends in _dev (a device
+ *     pointer from hor_dev_alloc).  Points are float64 xyz interleaved (3*n),
+ *     as the reference's Surf_point.x / Target_point and its SL[3*i+k] output.
+ *   - Every function returns HOR_OK (0) or a negative HOR_E* code; the text of
+ *     the last error on the calling thread is in hor_last_error().
+ *   - Calls are synchronous: outputs are complete on return.
+ *   - A context (hor_ctx) owns a HIP stream and device buffers; one context per
+ *     host thread.  There is no global mutable state.
+ *   - There is no CPU fallback: without a usable gfx950 device every compute
+ *     entry point fails with HOR_E_NODEVICE.
+ */
+#ifndef HOR_H
+#define HOR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  HOR_OK = 0,
+  HOR_E_ARG = -1,       /* bad size / null pointer / bad enum */
+  HOR_E_HIP = -2,       /* HIP runtime error */
+  HOR_E_NODEVICE = -3,  /* no usable GPU */
+  HOR_E_NAN = -4,       /* non-finite value detected */
+  HOR_E_STATE = -5,     /* object used out of order */
+  HOR_E_NOMEM = -6
+};
+
+/* Operator kinds. */
+enum {
+  HOR_SL_ON = 0,   /* eval_Stokes_SL_onSurf   (stokeslets.cpp:141-180) */
+  HOR_SL_OFF = 1,  /* eval_Stokes_SL_offSurf  (stokeslets.cpp:12-86)   */
+  HOR_DL_ON = 2,   /* eval_Stokes_DL_onSurf   (stresslets.cpp:186-228) */
+  HOR_DL_OFF = 3   /* eval_Stokes_DL_offSurf  (stresslets.cpp:13-96)   */
+};
+
+/* Summation method. */
+enum {
+  HOR_DIRECT = 0,  /* O(N*M) direct sums (stokeslets.cpp / stresslets.cpp) */
+  HOR_TREE = 1     /* KITC treecode (KITC.cpp) */
+};
+
+/* Arithmetic of the pair sums. */
+enum {
+  HOR_FP64 = 0,    /* every pair in FP64 (the parity path) */
+  HOR_MIXED = 1    /* pairs with r/delta > 8 and treecode proxy interactions in
+                      FP32, FP64 accumulation; near pairs and subtraction in FP64 */
+};
+
+typedef struct hor_ctx hor_ctx;
+
+const char* hor_last_error(void);
+const char* hor_version(void);
+
+/* Number of visible devices (0 when none); does not create a context. */
+int hor_device_count(int* count);
+
+int hor_ctx_create(int device, hor_ctx** ctx);
+int hor_ctx_destroy(hor_ctx* ctx);
+/* Wall time of the last compute call on this context, in ms (HIP events on
+ * the context's stream), and the kernel time of the dominant kernel. */
+int hor_ctx_last_timing(hor_ctx* ctx, double* total_ms, double* kernel_ms);
+
+/* Treecode parameters (KITC.h:14-17 are compile-time there; runtime here).
+ * Defaults P=6, N0=1000, theta=0.6. */
+typedef struct {
+  int P;          /* barycentric degree, 1..12 */
+  int N0;         /* leaf size */
+  double theta;   /* MAC parameter */
+} hor_tree_params;
+
+/* ------------------------------------------------------------------------
+ * Plans: geometry-only set-up is done once (AoS->SoA, spatial sort, per-target
+ * regularization coefficients, treecode tree and interaction lists) and kept
+ * on the device; densities change per call.
+ *
+ * A plan binds one source surface to one target set for one operator kind.
+ *   sx, sn: 3*ns positions / unit normals; sa: ns quadrature weights.
+ *   tx: 3*nt target positions (for *_ON, nt <= ns and target i is source i,
+ *       as the reference assumes: stokeslets.cpp:167, KITC.cpp:1461,1678).
+ *   tb: nt signed distances b (OFF only), tx0/tn0: 3*nt nearest surface point
+ *       and its normal (OFF only; tx0 used by DL only).
+ *   ctr: root-box centre for HOR_TREE (KITC.cpp:45-60), ignored otherwise.
+ * ------------------------------------------------------------------------ */
+typedef struct hor_plan hor_plan;
+
+int hor_plan_create(hor_ctx* ctx, int kind, int method, int precision,
+                    double DEL, int ns, const double* sx, const double* sn,
+                    const double* sa, int nt, const double* tx, const double* tb,
+                    const double* tx0, const double* tn0, const double* ctr,
+                    const hor_tree_params* tree, hor_plan** plan);
+int hor_plan_destroy(hor_plan* plan);
+
+/* Evaluates the plan's operator.
+ *   dens: 3*ns source density (f for SL, g for DL), host.
+ *   d0:   3*nt density at each target's nearest point (f0 = nrst_f for SL_OFF,
+ *         g0 = nrst_g for DL_OFF), host; ignored for *_ON.
+ *   out:  3*nt result, host. */
+int hor_plan_eval(hor_plan* plan, const double* dens, const double* d0, double* out);
+
+/* Device-resident variant: all three pointers are device pointers
+ * (hor_dev_alloc); dens_dev 3*ns, d0_dev 3*nt or NULL, out_dev 3*nt. */
+int hor_plan_eval_dev(hor_plan* plan, const double* dens_dev, const double* d0_dev,
+                      double* out_dev);
+
+/* Pair interactions one evaluation performs (direct: nt*ns; tree: near pairs,
+ * proxy interactions and moment terms separately). */
+int hor_plan_stats(hor_plan* plan, double* direct_pairs, double* near_pairs,
+                   double* proxy_pairs, double* moment_terms);
+
+/* ------------------------------------------------------------------------
+ * One-shot drop-ins of the reference functions (plan, eval, destroy).
+ * ------------------------------------------------------------------------ */
+/* eval_Stokes_SL_offSurf (stokeslets.h:9-12); tn0/tf0 are nrst_Nrml / nrst_f. */
+int hor_sl_offsurf(hor_ctx* ctx, double DEL, int ns, const double* sx,
+                   const double* sn, const double* sa, const double* sf, int nt,
+                   const double* tx, const double* tb, const double* tn0,
+                   const double* tf0, double* out);
+/* eval_Stokes_SL_onSurf (stokeslets.h:20-23). */
+int hor_sl_onsurf(hor_ctx* ctx, double DEL, int ns, const double* sx,
+                  const double* sn, const double* sa, const double* sf, int nt,
+                  const double* tx, double* out);
+/* eval_Stokes_DL_offSurf (stresslets.h:9-12); tx0/tn0/tg0 are nrst_x/y/z,
+ * nrst_Nrml, nrst_g. */
+int hor_dl_offsurf(hor_ctx* ctx, double DEL, int ns, const double* sx,
+                   const double* sn, const double* sa, const double* sg, int nt,
+                   const double* tx, const double* tb, const double* tx0,
+                   const double* tn0, const double* tg0, double* out);
+/* eval_Stokes_DL_onSurf (stresslets.h:21-24). */
+int hor_dl_onsurf(hor_ctx* ctx, double DEL, int ns, const double* sx,
+                  const double* sn, const double* sa, const double* sg, int nt,
+                  const double* tx, double* out);
+/* eval_Stokes_{SL,DL}_{on,off}Surf_tree (KITC.h:99-118); ctr = centre of the
+ * source surface's root box (ctr1/ctr2 in the reference); tree may be NULL. */
+int hor_sl_onsurf_tree(hor_ctx* ctx, const double* ctr, const hor_tree_params* tree,
+                       double DEL, int ns, const double* sx, const double* sn,
+                       const double* sa, const double* sf, int nt,
+                       const double* tx, double* out);
+int hor_sl_offsurf_tree(hor_ctx* ctx, const double* ctr, const hor_tree_params* tree,
+                        double DEL, int ns, const double* sx, const double* sn,
+                        const double* sa, const double* sf, int nt,
+                        const double* tx, const double* tb, const double* tn0,
+                        const double* tf0, double* out);
+int hor_dl_onsurf_tree(hor_ctx* ctx, const double* ctr, const hor_tree_params* tree,
+                       double DEL, int ns, const double* sx, const double* sn,
+                       const double* sa, const double* sg, int nt,
+                       const double* tx, double* out);
+int hor_dl_offsurf_tree(hor_ctx* ctx, const double* ctr, const hor_tree_params* tree,
+                        double DEL, int ns, const double* sx, const double* sn,
+                        const double* sa, const double* sg, int nt,
+                        const double* tx, const double* tb, const double* tx0,
+                        const double* tn0, const double* tg0, double* out);
+
+/* ------------------------------------------------------------------------
+ * Two-surface second-kind solve (highreg_stokes_2surf/main.cpp:140-677).
+ * Unknown order: [surface 1 xyz interleaved, surface 2 xyz interleaved].
+ * For each surface k: quadrature x/nrml/area, centre ctr, and the nearest
+ * data of its points relative to the OTHER surface (nrst: 0 = far sentinel,
+ * x0, n0, b, i123 Monge indices), as Compute_nearest_data produces.
+ * ------------------------------------------------------------------------ */
+typedef struct {
+  int n;
+  const double* x;
+  const double* nrml;
+  const double* area;
+  double ctr[3];
+  const int* nrst;
+  const double* x0;
+  const double* n0;
+  const double* b;
+  const int* i123;
+} hor_surface_desc;
+
+typedef struct hor_solver hor_solver;
+
+int hor_solver_create(hor_ctx* ctx, double h, double DEL, int method, int precision,
+                      const hor_tree_params* tree1, const hor_tree_params* tree2,
+                      const hor_surface_desc* s1, const hor_surface_desc* s2,
+                      hor_solver** solver);
+int hor_solver_destroy(hor_solver* solver);
+/* n = 3*(n1+n2) */
+int hor_solver_size(hor_solver* solver, int* n);
+
+/* solve_IE_1step (main.cpp:541-677): Au = u - 2/3 [DL11+DL21; DL12+DL22](u),
+ * with g0 interpolated on the 6x6 Monge stencil (main.cpp:916-1046).  Host
+ * pointers; the _dev variant takes device pointers of length n. */
+int hor_solver_matvec(hor_solver* solver, const double* u, double* Au);
+int hor_solver_matvec_dev(hor_solver* solver, const double* u_dev, double* Au_dev);
+
+/* eval_SL (main.cpp:421-537): sl = [SL11+SL21; SL12+SL22](f); f1/f2 are the
+ * surface forces (3*n_k) and f01/f02 the force at each point's nearest point
+ * on the other surface (nrst_f).  Host pointers. */
+int hor_solver_eval_sl(hor_solver* solver, const double* f1, const double* f2,
+                       const double* f01, const double* f02, double* sl);
+
+/* Interpolated g0 (main.cpp:916-1046) at the points of surface k (1 or 2)
+ * from the density u of the other surface; host pointers, out 3*n_k. */
+int hor_solver_interp_g0(hor_solver* solver, int k, const double* u, double* g0);
+
+/* Restarted GMRES(m) (main.cpp:210-416), device-resident, x0 = 0, restart from
+ * the true residual b - Ax.  b, x host pointers of length n.  hist (may be
+ * NULL) receives up to maxit relative residual estimates.  On return *iters is
+ * the iteration count and *relres the final estimate |g_{j+1}|/|b|. */
+int hor_solver_gmres(hor_solver* solver, const double* b, double* x, int m,
+                     double tol, int maxit, int* iters, double* relres,
+                     double* hist);
+
+/* ------------------------------------------------------------------------
+ * Device memory helpers (for callers who keep data resident).
+ * ------------------------------------------------------------------------ */
+int hor_dev_alloc(hor_ctx* ctx, size_t bytes, void** ptr);
+int hor_dev_free(hor_ctx* ctx, void* ptr);
+int hor_dev_upload(hor_ctx* ctx, void* dst_dev, const void* src, size_t bytes);
+int hor_dev_download(hor_ctx* ctx, void* dst, const void* src_dev, size_t bytes);
+
+/* ------------------------------------------------------------------------
+ * Host geometry (input producers; run once per geometry, on the CPU).
+ * ------------------------------------------------------------------------ */
+/* Level-set body: ellipsoid with semi-axes abc, centre ctr, un-rotation matrix
+ * R (row-major 3x3, x_body = R (x - ctr); identity if NULL), as the
+ * reference's phi/Dphi (highreg_stokes_2surf/utilities.cpp:16-158). */
+typedef struct {
+  double abc[3];
+  double ctr[3];
+  double R[9];
+  int rotated;
+} hor_body;
+
+int hor_body_init(hor_body* body, const double* abc, const double* ctr, const double* R);
+
+/* Generate_Surface (surface.cpp:13-71): intersections of the grid lines
+ * (multiples of h through the body centre) with the body, partition-of-unity
+ * weights with angle pou_deg (70 in the reference).  Returns the count in *n;
+ * arrays filled when cap >= *n (x, nrml: 3n; area: n; plane: n). */
+int hor_surface_generate(const hor_body* body, double h, double pou_deg, int cap,
+                         int* n, double* x, double* nrml, double* area, int* plane);
+
+/* Compute_nearest_data (surface.cpp:450-526) of targets against a surface of
+ * the body: nrst (0 when |b|/DEL > 8), x0, n0, b, i123.  The surface force
+ * (surfc_force, surface.cpp:530-559) at x0 is returned in f0 when non-NULL,
+ * zero for far targets. */
+int hor_nearest_data(const hor_body* body, double h, double DEL, int ns,
+                     const double* sx, const int* splane, int nt, const double* tx,
+                     int* nrst, double* x0, double* n0, double* b, int* i123,
+                     double* f0);
+
+/* surfc_force (surface.cpp:530-559): f = 2 H gamma n - grad_s gamma with
+ * gamma = 1 + z^2 (z relative to the centre) and H the mean curvature. */
+int hor_surface_force(const hor_body* body, int n, const double* x,
+                      const double* nrml, double* f);
+
+/* Grid nodes origin + h*(i,j,k), 0 <= i,j,k <= ngrid, with |b| <= band
+ * (find_nrst_surfc, extend_to_grid/surface.cpp:303-392): returns the count;
+ * fills when cap >= count: ijk (3n), x (3n), b, x0 (3n), n0 (3n). */
+int hor_band_targets(const hor_body* body, const double* origin, double h,
+                     int ngrid, double band, int ns, const double* sx, int cap,
+                     int* count, int* ijk, double* x, double* b, double* x0,
+                     double* n0);
+
+/* Translating-body densities and exact velocity.
+ * hor_traction_translating: single-layer density of a body translating with
+ * unit velocity U (for an ellipsoid: F/(4 pi abc) / sqrt(x^2/a^4+...)), which
+ * makes SL[f] = U inside and on the surface.
+ * hor_spheroid_exact_velocity: exterior velocity of a prolate spheroid
+ * (a > b = c) translating with U = (1,0,0) along its axis
+ * (extend_to_grid/utilities.cpp:360-411, Chwang & Wu). */
+int hor_traction_translating(const hor_body* body, const double* U, int n,
+                             const double* x, double* f);
+int hor_spheroid_exact_velocity(const hor_body* body, int n, const double* x,
+                                double* u);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

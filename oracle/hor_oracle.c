@@ -20,16 +20,25 @@ static double rootpi(void) { return sqrt(ORC_PI); }
  * Scalar kernels
  * ====================================================================== */
 
-/* evaluate_In_024 (highreg_stokes_2surf/utilities.cpp:327-337) followed by
- * get_reg_coeff_7ord (utilities.cpp:163-175): lambda = b/delta -> c1,c2,c3. */
-void orc_reg_coeff(double lam, double c[3]) {
+/* evaluate_In_024 (highreg_stokes_2surf/utilities.cpp:327-337):
+ * I_{2k}(lambda) = int_{|lambda|}^inf (s^2 - lambda^2)^k erfc(s) ds, k = 0,1,2,
+ * in closed form. */
+void orc_In_024(double lam, double I[3]) {
   const double rp = rootpi();
   double lam2 = lam * lam;
   double em2 = exp(-lam2);
   double t = fabs(lam) * erfc(fabs(lam));
-  double I0 = em2 / rp - t;
-  double I2 = 2.0 / 3.0 * ((0.5 - lam2) * em2 / rp + lam2 * t);
-  double I4 = 8.0 / 15.0 * ((0.75 - 0.5 * lam2 + lam2 * lam2) * em2 / rp - lam2 * lam2 * t);
+  I[0] = em2 / rp - t;
+  I[1] = 2.0 / 3.0 * ((0.5 - lam2) * em2 / rp + lam2 * t);
+  I[2] = 8.0 / 15.0 * ((0.75 - 0.5 * lam2 + lam2 * lam2) * em2 / rp - lam2 * lam2 * t);
+}
+
+/* get_reg_coeff_7ord (utilities.cpp:163-175): lambda = b/delta -> c1,c2,c3. */
+void orc_reg_coeff(double lam, double c[3]) {
+  double lam2 = lam * lam;
+  double I[3];
+  orc_In_024(lam, I);
+  double I0 = I[0], I2 = I[1], I4 = I[2];
   double e2 = exp(lam2);
   double c3 = e2 * (2.0 * I0 - 4.0 * I2 + I4) / 8.0;
   double c2 = e2 * (I0 - I2) + (4.0 * lam2 + 7.0) * c3;

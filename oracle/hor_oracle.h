@@ -32,6 +32,7 @@ extern "C" {
 #endif
 
 /* ---- scalar kernels (highreg_stokes_2surf/utilities.cpp:163-337) ---- */
+void orc_In_024(double lam, double I[3]);
 void orc_reg_coeff(double lam, double c[3]);
 void orc_sl_kernel(double r, double d, const double c[3], double* s1, double* s2);
 void orc_sl_kernel_on(double r, double d, double* s1, double* s2);

@@ -70,13 +70,18 @@ class Oracle:
         for name in ("orc_sl_off", "orc_sl_on", "orc_dl_off", "orc_dl_on",
                      "orc_sl_on_tree", "orc_sl_off_tree", "orc_dl_on_tree",
                      "orc_dl_off_tree", "orc_stencil", "orc_interp_g0",
-                     "orc_matvec", "orc_eval_sl", "orc_reg_coeff"):
+                     "orc_matvec", "orc_eval_sl", "orc_reg_coeff", "orc_In_024"):
             getattr(L, name).restype = None
         L.orc_gmres.restype = _I
         L.orc_tree_structure.restype = _I
         L.orc_tree_moments.restype = _I
 
     # ---- kernels
+    def In_024(self, lam):
+        out = np.zeros(3)
+        self.lib.orc_In_024(_D(lam), _p(out))
+        return out
+
     def reg_coeff(self, lam):
         out = np.zeros(3)
         self.lib.orc_reg_coeff(_D(lam), _p(out))

@@ -1,0 +1,90 @@
+"""GPU parity of the KITC treecode against the CPU oracle's treecode.
+
+Same tree, same MAC decisions, same leaf test: the FP64 GPU path agrees with
+the oracle to ||.||_2 relative 1e-12 (max-norm relative 1e-12).  The mixed
+path (FP32 proxies and far pairs) is held to 1e-6 against FP64.
+"""
+import numpy as np
+import pytest
+
+import hor
+from cases import ellipsoid_case, spheroid_pair
+from conftest import rel_l2, rel_max
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+def _check(a, b, tol=TOL):
+    assert np.all(np.isfinite(a))
+    assert rel_l2(a, b) <= tol, rel_l2(a, b)
+    assert rel_max(a, b) <= tol, rel_max(a, b)
+
+
+@pytest.fixture(scope="module")
+def ell():
+    return ellipsoid_case(h=1 / 16, max_band=2000)
+
+
+@pytest.mark.parametrize("P,N0,theta", [(6, 200, 0.6), (8, 300, 0.6), (4, 150, 0.8)])
+def test_tree_four_ops(ctx, oracle, ell, P, N0, theta):
+    body, s, t, DEL = ell
+    ctr = body.ctr
+    tp_o = oracle.tree_params(P, N0, theta, ctr)
+    tp_g = hor.TreeParams(P, N0, theta)
+    print("tree", P, N0, theta, s["n"], t["n"], flush=True)
+    _check(hor.ops.eval_Stokes_SL_onSurf_tree(ctx, ctr, 0, DEL, s, tree=tp_g), oracle.sl_on_tree(tp_o, DEL, s))
+    _check(hor.ops.eval_Stokes_DL_onSurf_tree(ctx, ctr, 0, DEL, s, tree=tp_g), oracle.dl_on_tree(tp_o, DEL, s))
+    _check(hor.ops.eval_Stokes_SL_offSurf_tree(ctx, ctr, 0, DEL, s, t, tree=tp_g),
+           oracle.sl_off_tree(tp_o, DEL, s, t))
+    _check(hor.ops.eval_Stokes_DL_offSurf_tree(ctx, ctr, 0, DEL, s, t, tree=tp_g),
+           oracle.dl_off_tree(tp_o, DEL, s, t))
+
+
+def test_tree_reference_defaults_spheroid_pair(ctx, oracle):
+    """P=6, N0=1000, theta=0.6 (KITC.h:14-17) on the reference's geometry at h=1/32,
+    including far targets (nrst = 0) and the treecode's jump rule."""
+    h, DEL, (b1, b2), (s1, s2), (t1, t2) = spheroid_pair(N=32)
+    rng = np.random.default_rng(11)
+    s1["g"] = rng.uniform(-1, 1, 3 * s1["n"])
+    t2["g0"] = np.where(np.repeat(t2["nrst"] != 0, 3), rng.uniform(-1, 1, 3 * t2["n"]), 0.0)
+    tp_o = oracle.tree_params(6, 1000, 0.6, b1.ctr)
+    _check(hor.ops.eval_Stokes_SL_offSurf_tree(ctx, b1.ctr, h, DEL, s1, t2), oracle.sl_off_tree(tp_o, DEL, s1, t2))
+    _check(hor.ops.eval_Stokes_DL_offSurf_tree(ctx, b1.ctr, h, DEL, s1, t2), oracle.dl_off_tree(tp_o, DEL, s1, t2))
+    _check(hor.ops.eval_Stokes_SL_onSurf_tree(ctx, b1.ctr, h, DEL, s1), oracle.sl_on_tree(tp_o, DEL, s1))
+    _check(hor.ops.eval_Stokes_DL_onSurf_tree(ctx, b1.ctr, h, DEL, s1), oracle.dl_on_tree(tp_o, DEL, s1))
+
+
+def test_tree_single_leaf_equals_direct(ctx, ell):
+    """With N0 > N the root is one leaf: the treecode is the direct sum."""
+    body, s, t, DEL = ell
+    tp = hor.TreeParams(6, 10 * s["n"], 0.6)
+    a = hor.ops.eval_Stokes_DL_offSurf_tree(ctx, body.ctr, 0, DEL, s, t, tree=tp)
+    b = hor.ops.eval_Stokes_DL_offSurf(ctx, 0, DEL, s, t)
+    # the direct DL adds g0 only in its regularized branch, the treecode for b<0 always
+    _check(a, b)
+
+
+def test_tree_mixed_vs_fp64(ctx, ell):
+    body, s, t, DEL = ell
+    tp = hor.TreeParams(8, 300, 0.6)
+    for kind, dens, d0 in ((hor.SL_OFF, s["f"], t["f0"]), (hor.DL_OFF, s["g"], t["g0"]),
+                           (hor.DL_ON, s["g"], None)):
+        a = hor.Plan(ctx, kind, s, None if kind == hor.DL_ON else t, DEL, hor.TREE, hor.MIXED, body.ctr, tp)(dens, d0)
+        b = hor.Plan(ctx, kind, s, None if kind == hor.DL_ON else t, DEL, hor.TREE, hor.FP64, body.ctr, tp)(dens, d0)
+        print("tree mixed vs fp64", kind, rel_l2(a, b), flush=True)
+        assert rel_l2(a, b) <= 1e-6
+
+
+def test_tree_converges_in_P(ctx, ell):
+    """The treecode converges as P grows (self-convergence against P = 10).
+    Its limit is not the regularized direct sum: the far field always uses the
+    plain kernel (KITC.cpp:1089-1101), also for accepted clusters within
+    8 delta, which leaves a P-independent gap at this coarse h."""
+    body, s, t, DEL = ell
+    ref = hor.ops.eval_Stokes_SL_offSurf_tree(ctx, body.ctr, 0, DEL, s, t, tree=hor.TreeParams(10, 150, 0.6))
+    errs = [rel_l2(hor.ops.eval_Stokes_SL_offSurf_tree(ctx, body.ctr, 0, DEL, s, t, tree=hor.TreeParams(P, 150, 0.6)), ref)
+            for P in (2, 4, 6, 8)]
+    print("tree vs P=10 by P", errs, flush=True)
+    assert all(errs[k + 1] < errs[k] for k in range(3))
+    assert errs[-1] < 1e-5
