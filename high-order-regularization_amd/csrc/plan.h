@@ -14,6 +14,7 @@ struct TreePlan;  // kitc.hip
 struct hor_plan {
   hor_ctx* ctx = nullptr;
   int kind = 0, method = 0, precision = 0;
+  bool all_near = false, on_at = false;  // HOR_ALL_NEAR / HOR_ON_AT
   double DEL = 0;
   int ns = 0, nsp = 0, nt = 0, ntp = 0;
   int ntile = 0, tiles_per_chunk = 0, nchunk = 0;
@@ -35,7 +36,7 @@ struct hor_plan {
 
 namespace hor {
 
-bool target_is_near(int kind, int method, double b, double DEL);
+bool target_is_near(const hor_plan* P, double b);
 void target_coefs(int kind, bool nearp, double b, double DEL, double q[kNumCoef]);
 int setup_targets(hor_plan* P, const double* tx, const double* tb, const double* tx0, const double* tn0,
                   const double* sx_orig);

@@ -110,9 +110,11 @@ static bool all_finite(const double* a, size_t n) {
 // Regularization class of a target: the reference's target-level test
 // (stokeslets.cpp:40 for SL, stresslets.cpp:41 for DL; on the surface the
 // regularized kernel is always used).
-bool target_is_near(int kind, int method, double b, double DEL) {
-  if (kind == HOR_SL_ON || kind == HOR_DL_ON) return true;
-  if (kind == HOR_SL_OFF && method == HOR_DIRECT) return !(std::fabs(b / DEL) > 8.0);
+bool target_is_near(const hor_plan* P, double b) {
+  int kind = P->kind;
+  double DEL = P->DEL;
+  if (kind == HOR_SL_ON || kind == HOR_DL_ON || P->all_near) return true;
+  if (kind == HOR_SL_OFF && P->method == HOR_DIRECT) return !(std::fabs(b / DEL) > 8.0);
   return !(std::fabs(b) / DEL > 8.0);  // stresslets.cpp:41, KITC.cpp:856, 970
 }
 
@@ -141,7 +143,7 @@ int setup_targets(hor_plan* P, const double* tx, const double* tb, const double*
   hipStream_t s = P->ctx->stream;
   int nt = P->nt, kind = P->kind;
   std::vector<int> cls(nt);
-  for (int i = 0; i < nt; i++) cls[i] = target_is_near(kind, P->method, tb ? tb[i] : 0.0, P->DEL) ? 1 : 0;
+  for (int i = 0; i < nt; i++) cls[i] = target_is_near(P, tb ? tb[i] : 0.0) ? 1 : 0;
   P->tperm_h = spatial_order(nt, tx, cls.data());
   int ntp = std::max(kBlock, round_up(nt, kBlock));
   P->ntp = ntp;
@@ -282,6 +284,7 @@ double op_scale(int kind) {
 int prepare_targets(hor_plan* P, const double* dens_dev, const double* d0_dev) {
   hipStream_t s = P->ctx->stream;
   int mode = (P->kind == HOR_SL_OFF) ? 0 : (P->kind == HOR_SL_ON) ? 1 : (P->kind == HOR_DL_OFF) ? 2 : 3;
+  if (P->on_at) mode = 4;  // on-surface stresslet at given targets: g0 = d0
   HOR_TRY(launch_target_sub(mode, P->nt, P->ntp, P->tperm.p, P->tnear.p, d0_dev, P->n0_orig.p, dens_dev,
                             P->sn_orig.p, P->tsub.p, s));
   return HOR_OK;
@@ -337,13 +340,19 @@ extern "C" int hor_plan_create(hor_ctx* ctx, int kind, int method, int precision
                                hor_plan** plan) {
   if (!ctx || !plan) return set_error(HOR_E_ARG, "hor_plan_create: null ctx/plan");
   *plan = nullptr;
+  const int flags = kind & 0xF0;
+  kind &= 0x0F;
   if (kind < HOR_SL_ON || kind > HOR_DL_OFF) return set_error(HOR_E_ARG, "bad kind %d", kind);
+  if ((flags & ~(HOR_ALL_NEAR | HOR_ON_AT)) || ((flags & HOR_ALL_NEAR) && kind != HOR_SL_OFF && kind != HOR_DL_OFF) ||
+      ((flags & HOR_ON_AT) && kind != HOR_DL_ON))
+    return set_error(HOR_E_ARG, "bad kind flags 0x%x for kind %d", flags, kind);
+  if (flags && method != HOR_DIRECT) return set_error(HOR_E_ARG, "kind flags apply to direct sums only");
   if (method != HOR_DIRECT && method != HOR_TREE) return set_error(HOR_E_ARG, "bad method %d", method);
   if (precision != HOR_FP64 && precision != HOR_MIXED) return set_error(HOR_E_ARG, "bad precision %d", precision);
   if (!(DEL > 0) || !std::isfinite(DEL)) return set_error(HOR_E_ARG, "DEL must be finite and > 0");
   if (ns <= 0 || nt <= 0) return set_error(HOR_E_ARG, "need ns > 0 and nt > 0 (ns=%d nt=%d)", ns, nt);
   if (!sx || !sn || !sa || !tx) return set_error(HOR_E_ARG, "null source/target array");
-  bool on = (kind == HOR_SL_ON || kind == HOR_DL_ON);
+  bool on = (kind == HOR_SL_ON || (kind == HOR_DL_ON && !(flags & HOR_ON_AT)));
   if (on && nt > ns) return set_error(HOR_E_ARG, "on-surface operator needs nt <= ns (target i is source i)");
   if (!on && (!tb || !tn0)) return set_error(HOR_E_ARG, "off-surface operator needs b and nrst normal");
   if (kind == HOR_DL_OFF && !tx0) return set_error(HOR_E_ARG, "DL off-surface operator needs nrst point");
@@ -356,6 +365,8 @@ extern "C" int hor_plan_create(hor_ctx* ctx, int kind, int method, int precision
   std::unique_ptr<hor_plan> P(new hor_plan());
   P->ctx = ctx;
   P->kind = kind;
+  P->all_near = (flags & HOR_ALL_NEAR) != 0;
+  P->on_at = (flags & HOR_ON_AT) != 0;
   P->method = method;
   P->precision = precision;
   P->DEL = DEL;
@@ -384,7 +395,7 @@ extern "C" int hor_plan_destroy(hor_plan* P) {
 
 extern "C" int hor_plan_eval_dev(hor_plan* P, const double* dens_dev, const double* d0_dev, double* out_dev) {
   if (!P || !dens_dev || !out_dev) return set_error(HOR_E_ARG, "hor_plan_eval_dev: null");
-  bool on = (P->kind == HOR_SL_ON || P->kind == HOR_DL_ON);
+  bool on = (P->kind == HOR_SL_ON || P->kind == HOR_DL_ON) && !P->on_at;
   if (!on && !d0_dev) return set_error(HOR_E_ARG, "off-surface operator needs d0");
   hor_ctx* c = P->ctx;
   HOR_HIP_TRY(hipSetDevice(c->device));
@@ -402,7 +413,7 @@ extern "C" int hor_plan_eval_dev(hor_plan* P, const double* dens_dev, const doub
 
 extern "C" int hor_plan_eval(hor_plan* P, const double* dens, const double* d0, double* out) {
   if (!P || !dens || !out) return set_error(HOR_E_ARG, "hor_plan_eval: null");
-  bool on = (P->kind == HOR_SL_ON || P->kind == HOR_DL_ON);
+  bool on = (P->kind == HOR_SL_ON || P->kind == HOR_DL_ON) && !P->on_at;
   if (!on && !d0) return set_error(HOR_E_ARG, "off-surface operator needs d0 (nrst_f / nrst_g)");
   if (!all_finite(dens, 3 * (size_t)P->ns) || (!on && !all_finite(d0, 3 * (size_t)P->nt)))
     return set_error(HOR_E_NAN, "non-finite density");

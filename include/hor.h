@@ -46,6 +46,16 @@ enum {
   HOR_DL_OFF = 3   /* eval_Stokes_DL_offSurf  (stresslets.cpp:13-96)   */
 };
 
+/* Modifier bits OR-ed into the kind of hor_plan_create (the reference's
+ * per-target helpers called on their own):
+ *   HOR_ALL_NEAR (SL_OFF / DL_OFF): every target takes the regularized branch
+ *     whatever its |b|/delta, as stokes_SL_7ord / stokes_DL_7ord do
+ *     (stokeslets.cpp:94-135, stresslets.cpp:104-179);
+ *   HOR_ON_AT (DL_ON): the on-surface stresslet at arbitrary targets with the
+ *     subtracted density g0 given per target in d0, as stokes_DL_7ord_onSurf
+ *     (stresslets.cpp:236-279). */
+enum { HOR_ALL_NEAR = 0x10, HOR_ON_AT = 0x20 };
+
 /* Summation method. */
 enum {
   HOR_DIRECT = 0,  /* O(N*M) direct sums (stokeslets.cpp / stresslets.cpp) */

@@ -1,0 +1,102 @@
+"""The C++ drop-in (include/hor_dropin.h) on the GPU.
+
+1. dropin_check calls all twelve reference functions through libhor_dropin.so;
+   results match the CPU oracle to 1e-12 (relative, 2-norm and max-norm).
+2. The reference's own two-surface program (its main.cpp, surface.cpp and
+   utilities.cpp, unmodified) linked against libhor_dropin.so instead of its
+   stokeslets/stresslets/KITC sources reproduces the CPU reference program:
+   same GMRES iteration count, velocities within 1e-10.
+"""
+import os
+import re
+import struct
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+from cases import ellipsoid_case
+from conftest import ROOT, rel_l2, rel_max
+
+pytestmark = pytest.mark.gpu
+DROPIN = os.path.join(ROOT, "high-order-regularization_amd", "dropin", "dropin_check")
+REF_MAIN = os.path.join(ROOT, "oracle", "_ref", "ref2surf_main")
+REF_MAIN_HOR = os.path.join(ROOT, "oracle", "_ref", "ref2surf_main_hor")
+
+
+def _check(a, b, tol=1e-12):
+    assert rel_l2(a, b) <= tol and rel_max(a, b) <= tol, (rel_l2(a, b), rel_max(a, b))
+
+
+def test_dropin_functions_match_oracle(oracle, tmp_path):
+    if not os.path.exists(DROPIN):
+        pytest.skip("dropin_check not built")
+    body, s, _, DEL = ellipsoid_case(h=1 / 16, max_band=10)
+    nq = s["n"]
+    nt = 300
+    P, N0, theta = 6, 300, 0.6
+    ctr = body.ctr
+    # targets: the first nt quadrature points, each its own nearest point (b = 0)
+    x = s["x"].reshape(-1, 3)
+    n = s["nrml"].reshape(-1, 3)
+    f = s["f"].reshape(-1, 3)
+    g = s["g"].reshape(-1, 3)
+    with open(tmp_path / "in.bin", "wb") as fh:
+        fh.write(struct.pack("<4i", nq, nt, P, N0))
+        fh.write(struct.pack("<5d", DEL, theta, *ctr))
+        for j in range(nq):
+            fh.write(np.concatenate([x[j], f[j], g[j], n[j], [s["area"][j]]]).astype("<f8").tobytes())
+        for i in range(nt):
+            fh.write(np.concatenate([x[i], x[i], n[i], f[i], g[i], [0.0]]).astype("<f8").tobytes())
+    subprocess.run([DROPIN, str(tmp_path / "in.bin"), str(tmp_path / "out.bin")], check=True, timeout=120)
+    out = np.fromfile(tmp_path / "out.bin", dtype="<f8")
+    blocks = out[: 8 * 3 * nt].reshape(8, 3 * nt)
+    per = out[8 * 3 * nt:].reshape(8, 4, 3)
+    tgt = dict(n=nt, x=s["x"][: 3 * nt], b=np.zeros(nt), x0=s["x"][: 3 * nt], n0=s["nrml"][: 3 * nt],
+               f0=s["f"][: 3 * nt], g0=s["g"][: 3 * nt])
+    tp = oracle.tree_params(P, N0, theta, ctr)
+    sl_on = oracle.sl_on(DEL, s, nt)
+    dl_on = oracle.dl_on(DEL, s, nt)
+    sl_off = oracle.sl_off(DEL, s, tgt)
+    dl_off = oracle.dl_off(DEL, s, tgt)
+    _check(blocks[0], sl_off)
+    _check(blocks[1], sl_on)
+    _check(blocks[2], dl_off)
+    _check(blocks[3], dl_on)
+    full_sl_on_tree = oracle.sl_on_tree(tp, DEL, s)[: 3 * nt]
+    full_dl_on_tree = oracle.dl_on_tree(tp, DEL, s)[: 3 * nt]
+    _check(blocks[4], full_sl_on_tree)
+    _check(blocks[5], oracle.sl_off_tree(tp, DEL, s, tgt))
+    _check(blocks[6], full_dl_on_tree)
+    _check(blocks[7], oracle.dl_off_tree(tp, DEL, s, tgt))
+    # per-target helpers on the first 8 targets
+    _check(per[:, 0].ravel(), sl_off[:24])   # stokes_SL_7ord with b = 0
+    _check(per[:, 1].ravel(), sl_on[:24])    # stokes_SL_7ord_onSurf
+    _check(per[:, 2].ravel(), dl_off[:24])   # stokes_DL_7ord (b = 0 adds g0)
+    _check(per[:, 3].ravel(), dl_on[:24])    # stokes_DL_7ord_onSurf
+
+
+def _run_main(exe, cwd):
+    p = subprocess.run([exe], cwd=cwd, capture_output=True, text=True, timeout=300)
+    assert p.returncode == 0, p.stderr[-2000:]
+    m = re.search(r"GMRES converged in (\d+) iterations with residual = (\S+)", p.stdout)
+    assert m, p.stdout[-2000:]
+    v1 = np.loadtxt(os.path.join(cwd, "vel1_N32_highreg_57_k4"))
+    v2 = np.loadtxt(os.path.join(cwd, "vel2_N32_highreg_57_k4"))
+    t = re.search(r"CPU time elapsed: (\S+) seconds", p.stdout)
+    return int(m.group(1)), float(m.group(2)), np.concatenate([v1.ravel(), v2.ravel()]), float(t.group(1))
+
+
+def test_reference_program_on_dropin():
+    if not (os.path.exists(REF_MAIN) and os.path.exists(REF_MAIN_HOR)):
+        pytest.skip("reference programs (oracle/_ref) not built")
+    with tempfile.TemporaryDirectory() as a, tempfile.TemporaryDirectory() as b:
+        print("running the reference two-surface program (CPU) and the same program on the drop-in", flush=True)
+        it_c, res_c, u_c, t_c = _run_main(REF_MAIN, a)
+        it_g, res_g, u_g, t_g = _run_main(REF_MAIN_HOR, b)
+    print(f"CPU reference: {it_c} its, res {res_c:.3e}, {t_c:.2f}s; drop-in: {it_g} its, res {res_g:.3e}, {t_g:.2f}s",
+          flush=True)
+    assert it_g == it_c
+    assert res_g <= 1e-10
+    assert rel_l2(u_g, u_c) <= 1e-10
