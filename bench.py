@@ -108,6 +108,91 @@ def cpu_baseline(s, t, DEL, target_seconds=15.0):
             "sample": f"{n} of {t['n']} band targets x {s['n']} sources, SL_OFF + DL_OFF, {dt:.1f}s"}
 
 
+def two_sphere_problem(h, gap, seed=SEED):
+    """Two spheres of radius 0.5, centres (0,0,+-(1+gap)/2), the surface
+    dicts of the two-surface solver (nearest data of each surface's points
+    on the other surface) and two right-hand sides:
+      rigid:   seeded rigid-body data b = U_k + W_k x (x - c_k), U, W ~ U(-1,1);
+      forcing: the reference's b = -2/3 SL[f] with its surface force f
+               (main.cpp:199-207, surface.cpp:530-559)."""
+    from hor import geometry as G
+    DEL = G.delta(h)
+    zc = 0.5 + gap / 2
+    bodies = [G.Body((0.5,) * 3, (0.0, 0.0, -zc)), G.Body((0.5,) * 3, (0.0, 0.0, zc))]
+    surfs = [b.surface(h) for b in bodies]
+    pd = []
+    for k in range(2):
+        t = bodies[1 - k].nearest(h, DEL, surfs[1 - k], surfs[k]["x"], with_force=True)
+        pd.append(dict(surfs[k], ctr=bodies[k].ctr, nrst=t["nrst"], x0=t["x0"], n0=t["n0"], b=t["b"],
+                       i123=t["i123"], f=bodies[k].force(surfs[k]["x"], surfs[k]["nrml"]), f0=t["f0"]))
+    rng = np.random.default_rng(seed)
+    rigid = []
+    for k in range(2):
+        U, Wv = rng.uniform(-1, 1, 3), rng.uniform(-1, 1, 3)
+        x = pd[k]["x"].reshape(-1, 3)
+        rigid.append((U + np.cross(Wv, x - bodies[k].ctr)).ravel())
+    return DEL, bodies, pd, np.concatenate(rigid)
+
+
+def gmres_workload(h=1.0 / 128.0, gap=0.1, P=8, N0=4000, theta=0.6, tol=1e-8, seed=SEED,
+                   direct_check=False):
+    """Second-kind double-layer equation A u = u - 2/3 DL u = b on two spheres
+    (two_sphere_problem), treecode matvec, GMRES(15) to relative residual tol,
+    FP64 and mixed-precision treecodes, both right-hand sides.  Every solution's
+    residual is rechecked with the FP64 treecode operator (and, with
+    direct_check, with the FP64 direct-sum operator)."""
+    import hor
+    from hor.solver import Solver
+    t0 = time.time()
+    DEL, bodies, pd, b_rigid = two_sphere_problem(h, gap, seed)
+    geo_s = time.time() - t0
+    ctx = hor.Context(0)
+    tp = hor.TreeParams(P, N0, theta)
+    n = b_rigid.size
+    out = {"workload": f"two spheres r=0.5, gap {gap}, h=1/{round(1 / h)}, treecode P={P} N0={N0} "
+                       f"theta={theta}, GMRES(15) to {tol:g}",
+           "unknowns": int(n), "delta": DEL, "geometry_s": geo_s}
+    solvers = {}
+    for name, prec in (("fp64", hor.FP64), ("mixed", hor.MIXED)):
+        t0 = time.time()
+        solvers[name] = Solver(ctx, h, DEL, pd[0], pd[1], method=hor.TREE, precision=prec, tree1=tp, tree2=tp)
+        out.setdefault("setup_s", {})[name] = time.time() - t0
+    S64 = solvers["fp64"]
+    b_force = -(2.0 / 3.0) * S64.eval_sl(pd[0]["f"], pd[1]["f"], pd[0]["f0"], pd[1]["f0"])
+    for name, S in solvers.items():
+        S.matvec(b_rigid)  # warm-up
+        t0 = time.perf_counter()
+        for _ in range(3):
+            S.matvec(b_rigid)
+        out.setdefault("matvec_ms", {})[name] = 1e3 * (time.perf_counter() - t0) / 3
+    S_dir = Solver(ctx, h, DEL, pd[0], pd[1], method=hor.DIRECT, precision=hor.FP64) if direct_check else None
+    for rhs_name, b in (("rigid", b_rigid), ("forcing", b_force)):
+        res = {}
+        xs = {}
+        for name, S in solvers.items():
+            t0 = time.perf_counter()
+            x, it, hist, est = S.gmres(b, m=15, tol=tol, maxit=300)
+            dt = time.perf_counter() - t0
+            xs[name] = x
+            r = b - S64.matvec(x)
+            res[name] = {"iterations": it, "seconds": dt, "ms_per_iteration": 1e3 * dt / max(it, 1),
+                         "relres_estimate": est,
+                         "relres_fp64_recheck": float(np.linalg.norm(r) / np.linalg.norm(b))}
+            if S_dir is not None:
+                rd = b - S_dir.matvec(x)
+                res[name]["relres_fp64_direct"] = float(np.linalg.norm(rd) / np.linalg.norm(b))
+        res["mixed"]["rel_diff_vs_fp64_solution"] = float(np.linalg.norm(xs["mixed"] - xs["fp64"]) /
+                                                          np.linalg.norm(xs["fp64"]))
+        res["speedup_mixed_vs_fp64"] = res["fp64"]["seconds"] / res["mixed"]["seconds"]
+        out[rhs_name] = res
+    for S in solvers.values():
+        S.close()
+    if S_dir is not None:
+        S_dir.close()
+    ctx.close()
+    return out
+
+
 def profile_traffic():
     """HBM bytes per launch of the dominant kernel from the committed
     rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated)."""
@@ -124,6 +209,10 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-gmres", action="store_true", help="skip the two-sphere GMRES workload")
+    ap.add_argument("--no-h256", action="store_true", help="skip the h=1/256 two-sphere GMRES workload")
+    ap.add_argument("--gmres-only", type=int, default=0, metavar="N",
+                    help="run only the two-sphere GMRES workload at h=1/N (128 or 256) and print its JSON")
     ap.add_argument("--h", type=float, default=H)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -132,6 +221,12 @@ def main():
         sys.exit(2)
     import hor
 
+    if args.gmres_only:
+        if args.gmres_only == 128:
+            print(json.dumps(gmres_workload()), flush=True)
+        else:
+            print(json.dumps(gmres_workload(h=1.0 / args.gmres_only, P=10, N0=8000)), flush=True)
+        return
     body, s, t, DEL = build_workload(h=args.h)
     ctx = hor.Context(0)
     t0 = time.time()
@@ -203,6 +298,13 @@ def main():
             "near_pair_fraction": fnear}
 
     cpu = None if args.no_cpu else cpu_baseline(s, t, DEL)
+    gm = None
+    if not args.no_gmres:
+        log("two-sphere GMRES workload (h=1/128, treecode P=8 N0=4000)")
+        gm = {"h128": gmres_workload()}
+        if not args.no_h256:
+            log("two-sphere GMRES workload (h=1/256, treecode P=10 N0=8000)")
+            gm["h256"] = gmres_workload(h=1.0 / 256.0, P=10, N0=8000, direct_check=True)
     line = {
         "metric": "pair interactions/s",
         "value": value,
@@ -224,6 +326,7 @@ def main():
         "mixed": {"ms_per_step": mixed_ms, "value": pairs_per_step / (mixed_ms * 1e-3),
                   "rel_err_vs_fp64_sl": err_sl, "rel_err_vs_fp64_dl": err_dl},
         "accuracy": {"sl_max_err_inside_vs_exact_U": err_inside},
+        "gmres": gm,
     }
     for p in dev.values():
         ctx.free(p)

@@ -12,12 +12,15 @@ namespace hor {
 // plan's sorted order with stride ntp (a multiple of 256).
 struct DirectArgs {
   int ns, nsp, nt, ntp;
-  int ntile, tiles_per_chunk;      // 256-source LDS tiles, tiles per grid.y chunk
+  int nsub, subs_per_chunk;        // 64-source subtiles, subtiles per grid.y chunk
   double DEL, far_d2;              // delta, (8 delta (1+1e-10))^2
+  double near_r2;                  // (8 delta)^2 (1+1e-12): pairs beyond are plain for sure
   const double* sx;                // 3*nsp positions
   const double* sn;                // 3*nsp normals
   const double* sa;                // nsp weights
   const double* sd;                // 3*nsp density (f or g)
+  const struct SrcRec* rec;        // nsp source records (sorted), this evaluation's density
+  const struct SrcRec32* rec32;    // FP32 copies (mixed precision) or null
   const double* sub_box;           // 6 per 64-source subtile (lo xyz, hi xyz)
   const double* tx;                // 3*ntp positions
   const double* tcoef;             // 12*ntp regularization polynomial coefficients
@@ -34,7 +37,9 @@ int launch_direct(const DirectArgs& A, int kind, bool mixed, int nchunk, hipStre
 int launch_finalize(const double* partial, int nchunk, int nt, int ntp, double scale,
                     const double* xi, const double* g0, const int* tperm, double* out,
                     const double* base, double coef, int accumulate, hipStream_t s);
-int launch_gather3(const double* dens, const int* perm, int n, int np, double* sd, hipStream_t s);
+int launch_gather3(const double* dens, const int* perm, int n, int np, const double* sx, const double* sn,
+                   const double* sa, double* sd, struct SrcRec* rec, struct SrcRec32* rec32,
+                   int scale_density, hipStream_t s);
 int launch_target_sub(int mode, int nt, int ntp, const int* tperm, const int* tnear,
                       const double* d0, const double* n0_orig, const double* dens,
                       const double* sn_orig, double* out, hipStream_t s);

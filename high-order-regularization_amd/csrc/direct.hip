@@ -2,9 +2,11 @@
 //
 // One thread owns one target (targets sorted far-first, then by Morton key,
 // so a wave's 64 targets are spatially compact and share one regularization
-// class).  A workgroup of 256 targets streams the sources (Morton sorted)
-// through 256-entry LDS tiles; each wave walks a tile in 64-source subtiles
-// and, per subtile, takes one of two wave-uniform paths:
+// class).  Every lane of a wave visits the same source at the same time, so
+// the source records (Morton sorted, array of structures) stream through
+// wave-uniform scalar loads into SGPRs and feed the FP64 VALU directly: no LDS
+// traffic, no barriers.  A wave walks the sources in 64-source subtiles and,
+// per subtile, takes one of two wave-uniform paths:
 //   * fast path: the subtile's bounding box lies beyond 8*delta (plus a
 //     rounding margin) of the wave's target box, or every target of the wave
 //     is beyond 8*delta of the surface: every pair uses the plain kernel
@@ -20,8 +22,6 @@ namespace hor {
 
 template <int KIND, bool MIXED>
 __global__ __launch_bounds__(kBlock) void k_direct(DirectArgs A) {
-  __shared__ SrcRec tile[kBlock];
-  __shared__ SrcRec32 tile32[MIXED ? kBlock : 1];
   const int tid = threadIdx.x;
   const int i = blockIdx.x * kBlock + tid;     // sorted target (ntp is a multiple of kBlock)
   const int wave = i / kWave;
@@ -31,30 +31,10 @@ __global__ __launch_bounds__(kBlock) void k_direct(DirectArgs A) {
   const bool all_far = wb[6] != 0.0;
   float pxf = (float)t.px, pyf = (float)t.py, pzf = (float)t.pz;
   double u[3] = {0.0, 0.0, 0.0};
-  const int tile0 = blockIdx.y * A.tiles_per_chunk;
-  const int tile1 = min(A.ntile, tile0 + A.tiles_per_chunk);
-  for (int tl = tile0; tl < tile1; tl++) {
-    const int base = tl * kBlock;
-    const int len = min(kBlock, A.ns - base);
-    __syncthreads();
-    if (tid < len) {
-      int j = base + tid;
-      SrcRec r;
-      r.x = A.sx[j]; r.y = A.sx[A.nsp + j]; r.z = A.sx[2 * A.nsp + j]; r.a = A.sa[j];
-      r.nx = A.sn[j]; r.ny = A.sn[A.nsp + j]; r.nz = A.sn[2 * A.nsp + j]; r.pad0 = 0;
-      r.d0 = A.sd[j]; r.d1 = A.sd[A.nsp + j]; r.d2 = A.sd[2 * A.nsp + j]; r.pad1 = 0;
-      tile[tid] = r;
-      if (MIXED) {
-        SrcRec32 q;
-        q.x = (float)r.x; q.y = (float)r.y; q.z = (float)r.z; q.a = (float)r.a;
-        q.nx = (float)r.nx; q.ny = (float)r.ny; q.nz = (float)r.nz; q.pad0 = 0;
-        q.d0 = (float)r.d0; q.d1 = (float)r.d1; q.d2 = (float)r.d2; q.pad1 = 0;
-        tile32[tid] = q;
-      }
-    }
-    __syncthreads();
-    tile_sum<KIND, MIXED>(A, tile, tile32, len, base / kWave, wb, all_far, t, pxf, pyf, pzf, u);
-  }
+  // this chunk's 64-source subtiles
+  const int sub0 = blockIdx.y * A.subs_per_chunk;
+  const int sub1 = min(A.nsub, sub0 + A.subs_per_chunk);
+  range_sum<KIND, MIXED>(A, sub0 * kWave, min(A.ns, sub1 * kWave), sub0, wb, all_far, t, pxf, pyf, pzf, u);
   double* P = A.partial + (size_t)blockIdx.y * 3 * A.ntp;
   P[i] = u[0];
   P[A.ntp + i] = u[1];
@@ -84,14 +64,34 @@ __global__ void k_direct_finalize(const double* partial, int nchunk, int nt, int
   }
 }
 
-// Density gather into sorted SoA: sd[k*nsp + j] = dens[3*perm[j] + k].
-__global__ void k_gather3(const double* dens, const int* perm, int n, int np, double* sd) {
+// Per-evaluation source records: sorted AoS (position, weight, normal,
+// density) for the sum kernels, plus the density as sorted SoA for the
+// treecode moments: rec[j] <- geometry[j], dens[3*perm[j] + k].
+__global__ void k_gather3(const double* dens, const int* perm, int n, int np, const double* sx, const double* sn,
+                          const double* sa, double* sd, SrcRec* rec, SrcRec32* rec32, int scale_density) {
   int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
   int o = perm[j];
-  sd[j] = dens[3 * o];
-  sd[np + j] = dens[3 * o + 1];
-  sd[2 * np + j] = dens[3 * o + 2];
+  SrcRec r;
+  r.x = sx[j]; r.y = sx[np + j]; r.z = sx[2 * np + j]; r.a = sa[j];
+  r.nx = r.a * sn[j]; r.ny = r.a * sn[np + j]; r.nz = r.a * sn[2 * np + j]; r.pad0 = 0.0;
+  r.d0 = dens[3 * o]; r.d1 = dens[3 * o + 1]; r.d2 = dens[3 * o + 2]; r.pad1 = 0.0;
+  sd[j] = r.d0;
+  sd[np + j] = r.d1;
+  sd[2 * np + j] = r.d2;
+  if (scale_density) {
+    r.d0 *= r.a;
+    r.d1 *= r.a;
+    r.d2 *= r.a;
+  }
+  rec[j] = r;
+  if (rec32) {
+    SrcRec32 q;
+    q.x = (float)r.x; q.y = (float)r.y; q.z = (float)r.z; q.a = (float)r.a;
+    q.nx = (float)r.nx; q.ny = (float)r.ny; q.nz = (float)r.nz; q.pad0 = 0.0f;
+    q.d0 = (float)r.d0; q.d1 = (float)r.d1; q.d2 = (float)r.d2; q.pad1 = 0.0f;
+    rec32[j] = q;
+  }
 }
 
 // Per-target subtraction data (sorted order):
@@ -174,9 +174,12 @@ int launch_finalize(const double* partial, int nchunk, int nt, int ntp, double s
   return HOR_OK;
 }
 
-int launch_gather3(const double* dens, const int* perm, int n, int np, double* sd, hipStream_t s) {
+int launch_gather3(const double* dens, const int* perm, int n, int np, const double* sx, const double* sn,
+                   const double* sa, double* sd, SrcRec* rec, SrcRec32* rec32, int scale_density,
+                   hipStream_t s) {
   if (n == 0) return HOR_OK;
-  hipLaunchKernelGGL(k_gather3, dim3((n + 255) / 256), dim3(256), 0, s, dens, perm, n, np, sd);
+  hipLaunchKernelGGL(k_gather3, dim3((n + 255) / 256), dim3(256), 0, s, dens, perm, n, np, sx, sn, sa, sd, rec, rec32,
+                     scale_density);
   HOR_HIP_TRY(hipGetLastError());
   return HOR_OK;
 }

@@ -45,7 +45,10 @@ struct TreePlan {
   int nfar_nodes = 0, n_mom_items = 0;
   DBuf<int> mom_item_node, mom_item_s0, mom_item_s1, mom_node_first, mom_node_nitems, fac_off;
   DBuf<double> fac;          // per far node: A1 (np1*n), A2, A3, D (n)
-  DBuf<double> mom_part, mom;
+  DBuf<double> mom_part, mom;    // mom: per far node, proxy-major [kk][ncomp]
+  DBuf<float> mom32;             // mixed precision: [kk][ncomp32] FP32 copy (padded)
+  DBuf<float> cheb32;            // per node: Chebyshev points minus the node centre, FP32
+  DBuf<double> node_ctr;         // per node: centre (midpoint of the Chebyshev interval)
   // reduction
   DBuf<int> slot_start;      // caller order, nt+1
   DBuf<double> slots;        // 3*nslot
@@ -118,16 +121,21 @@ __global__ __launch_bounds__(kBlock) void k_moments(MomArgs A) {
   }
 }
 
-// Sums the member-chunk partials of each far node in chunk order.
+// Sums the member-chunk partials of each far node in chunk order and stores
+// the moments proxy-major (mom[f][kk][c]), so that the far-field kernel reads
+// one proxy's components with one wide scalar load; the mixed-precision copy
+// mom32[f][kk][c] is padded to ncomp32 floats per proxy.
 __global__ void k_moments_reduce(int nfar, int ncomp, int pflat, const int* node_first, const int* node_nitems,
-                                 const double* part, double* mom) {
+                                 const double* part, double* mom, float* mom32, int ncomp32) {
   int f = blockIdx.x;
   int total = ncomp * pflat;
   for (int e = threadIdx.x; e < total; e += blockDim.x) {
     double s = 0.0;
     int it0 = node_first[f], nit = node_nitems[f];
     for (int q = 0; q < nit; q++) s += part[(size_t)(it0 + q) * total + e];
-    mom[(size_t)f * total + e] = s;
+    const int c = e / pflat, kk = e - c * pflat;
+    mom[(size_t)f * total + (size_t)kk * ncomp + c] = s;
+    if (mom32) mom32[((size_t)f * pflat + kk) * ncomp32 + c] = (float)s;
   }
 }
 
@@ -151,8 +159,6 @@ __device__ __forceinline__ double wave_max(double v) {
 // Leaf direct sums (Call_Ds_*, KITC.cpp:807-1062) for up to 256 near targets.
 template <int KIND, bool MIXED>
 __global__ __launch_bounds__(kBlock) void k_leaf(LeafArgs A) {
-  __shared__ SrcRec tile[kBlock];
-  __shared__ SrcRec32 tile32[MIXED ? kBlock : 1];
   const int tid = threadIdx.x;
   const int item = blockIdx.x;
   const int node = A.item_node[item];
@@ -171,30 +177,7 @@ __global__ __launch_bounds__(kBlock) void k_leaf(LeafArgs A) {
   float pxf = (float)t.px, pyf = (float)t.py, pzf = (float)t.pz;
   double u[3] = {0.0, 0.0, 0.0};
   const int lo = A.node_lo[node], hi = A.node_hi[node];
-  const int sub0 = A.leaf_sub_off[node];
-  for (int base = lo; base <= hi; base += kBlock) {
-    const int len = min(kBlock, hi + 1 - base);
-    __syncthreads();
-    if (tid < len) {
-      int j = base + tid;
-      const DirectArgs& D = A.D;
-      SrcRec r;
-      r.x = D.sx[j]; r.y = D.sx[D.nsp + j]; r.z = D.sx[2 * D.nsp + j]; r.a = D.sa[j];
-      r.nx = D.sn[j]; r.ny = D.sn[D.nsp + j]; r.nz = D.sn[2 * D.nsp + j]; r.pad0 = 0;
-      r.d0 = D.sd[j]; r.d1 = D.sd[D.nsp + j]; r.d2 = D.sd[2 * D.nsp + j]; r.pad1 = 0;
-      tile[tid] = r;
-      if (MIXED) {
-        SrcRec32 q;
-        q.x = (float)r.x; q.y = (float)r.y; q.z = (float)r.z; q.a = (float)r.a;
-        q.nx = (float)r.nx; q.ny = (float)r.ny; q.nz = (float)r.nz; q.pad0 = 0;
-        q.d0 = (float)r.d0; q.d1 = (float)r.d1; q.d2 = (float)r.d2; q.pad1 = 0;
-        tile32[tid] = q;
-      }
-    }
-    __syncthreads();
-    tile_sum<KIND, MIXED>(A.D, tile, tile32, len, sub0 + (base - lo) / kWave, wb, all_far, t, pxf, pyf,
-                          pzf, u);
-  }
+  range_sum<KIND, MIXED>(A.D, lo, hi + 1, A.leaf_sub_off[node], wb, all_far, t, pxf, pyf, pzf, u);
   if (act) {
     int s = A.slot[li];
     A.slots[s] = u[0];
@@ -206,11 +189,24 @@ __global__ __launch_bounds__(kBlock) void k_leaf(LeafArgs A) {
 struct FarArgs {
   int np1, pflat, ntp, nslot;
   const int *item_node, *item_off, *item_cnt, *tgt, *slot, *far_idx;
-  const double *cheb, *mom, *tx, *tsub2;
+  const double *cheb, *mom, *tx, *tsub2, *node_ctr;
+  const float *cheb32, *mom32;
   double* slots;
 };
 
-// Call_Treecode_SL / Call_Treecode_DL (KITC.cpp:671-777).
+constexpr int kNc32Sl = 8;    // FP32 moment floats per proxy (6 used)
+constexpr int kNc32Dl = 16;   // (12 used)
+
+// Call_Treecode_SL / Call_Treecode_DL (KITC.cpp:671-777): plain Stokeslet /
+// stresslet from the (P+1)^3 proxies of one far node, with the target-side
+// subtraction folded in:
+//   SL: u += m / r + d (m.d) / r^3,             m = W_f - (f0.n0) W_n
+//   DL: u += d [d.M.d - (g0.d)(W_n.d)] / r^5,   M_lm = W_{g_l n_m}
+// (the DL form is the reference's sum_l d_l sum_m (W_lm - g0_l W_n,m) d_m).
+// One work item is a far node and up to 256 of its targets; the node's
+// proxies and moments are wave-uniform.  In mixed precision the proxy
+// interactions run in FP32 on node-centred coordinates, each row of P+1
+// proxies is summed in FP32 and the rows in FP64.
 template <bool DL, bool MIXED>
 __global__ __launch_bounds__(kBlock) void k_far(FarArgs A) {
   const int tid = threadIdx.x;
@@ -222,75 +218,111 @@ __global__ __launch_bounds__(kBlock) void k_far(FarArgs A) {
   const int i = A.tgt[li];
   const int np1 = A.np1, pflat = A.pflat;
   const double px = A.tx[i], py = A.tx[A.ntp + i], pz = A.tx[2 * A.ntp + i];
-  const double* t1 = A.cheb + (size_t)3 * np1 * node;
-  const double* t2 = t1 + np1;
-  const double* t3 = t2 + np1;
-  constexpr int NC = DL ? 12 : 6;
-  const double* M = A.mom + (size_t)A.far_idx[node] * NC * pflat;
+  const int fi = A.far_idx[node];
   double u0 = 0.0, u1 = 0.0, u2 = 0.0;
-  if (!DL) {
-    const double sub = A.tsub2[i];
-    int kk = 0;
-    for (int a = 0; a < np1; a++) {
-      double d0 = px - t1[a], ti = d0 * d0;
-      for (int b = 0; b < np1; b++) {
-        double d1 = py - t2[b], tj = d1 * d1;
-        for (int c = 0; c < np1; c++, kk++) {
-          double d2 = pz - t3[c];
-          double m0 = M[kk] - sub * M[3 * pflat + kk];
-          double m1 = M[pflat + kk] - sub * M[4 * pflat + kk];
-          double m2 = M[2 * pflat + kk] - sub * M[5 * pflat + kk];
-          if (MIXED) {
-            float fd0 = (float)d0, fd1 = (float)d1, fd2 = (float)d2;
-            float r2 = fd0 * fd0 + fd1 * fd1 + fd2 * fd2;
-            float ri = rsqrt32(r2);
-            float f0 = (float)m0, f1 = (float)m1, f2 = (float)m2;
-            float tmp = (f0 * fd0 + f1 * fd1 + f2 * fd2) * (ri * ri * ri);
-            u0 += (double)(f0 * ri + tmp * fd0);
-            u1 += (double)(f1 * ri + tmp * fd1);
-            u2 += (double)(f2 * ri + tmp * fd2);
-          } else {
-            double r2 = ti + tj + d2 * d2;
-            double ri = rsqrt64(r2);
-            double tmp = (m0 * d0 + m1 * d1 + m2 * d2) * (ri * ri * ri);
-            u0 += m0 * ri + tmp * d0;
-            u1 += m1 * ri + tmp * d1;
-            u2 += m2 * ri + tmp * d2;
+  if (MIXED) {
+    const float* t1 = A.cheb32 + (size_t)3 * np1 * node;
+    const float* t2 = t1 + np1;
+    const float* t3 = t2 + np1;
+    const double* ctr = A.node_ctr + 3 * node;
+    const float qx = (float)(px - ctr[0]), qy = (float)(py - ctr[1]), qz = (float)(pz - ctr[2]);
+    constexpr int NC = DL ? kNc32Dl : kNc32Sl;
+    const float* M = A.mom32 + (size_t)fi * pflat * NC;
+    if (!DL) {
+      const float sub = (float)A.tsub2[i];
+      for (int a = 0, kk = 0; a < np1; a++) {
+        const float d0 = qx - t1[a];
+        for (int b = 0; b < np1; b++) {
+          const float d1 = qy - t2[b];
+          const float r2ab = d0 * d0 + d1 * d1;
+          float v0 = 0.f, v1 = 0.f, v2 = 0.f;
+          for (int c = 0; c < np1; c++, kk++) {
+            const float* m = M + (size_t)kk * NC;
+            const float d2 = qz - t3[c];
+            const float ri = rsqrt32(__builtin_fmaf(d2, d2, r2ab));
+            const float m0 = m[0] - sub * m[3], m1 = m[1] - sub * m[4], m2 = m[2] - sub * m[5];
+            const float tq = (m0 * d0 + m1 * d1 + m2 * d2) * (ri * ri);
+            v0 += (m0 + tq * d0) * ri;
+            v1 += (m1 + tq * d1) * ri;
+            v2 += (m2 + tq * d2) * ri;
           }
+          u0 += (double)v0; u1 += (double)v1; u2 += (double)v2;
+        }
+      }
+    } else {
+      const float g0x = (float)A.tsub2[i], g0y = (float)A.tsub2[A.ntp + i], g0z = (float)A.tsub2[2 * A.ntp + i];
+      for (int a = 0, kk = 0; a < np1; a++) {
+        const float d0 = qx - t1[a];
+        for (int b = 0; b < np1; b++) {
+          const float d1 = qy - t2[b];
+          const float r2ab = d0 * d0 + d1 * d1;
+          const float gab = g0x * d0 + g0y * d1;
+          float v0 = 0.f, v1 = 0.f, v2 = 0.f;
+          for (int c = 0; c < np1; c++, kk++) {
+            const float* m = M + (size_t)kk * NC;
+            const float d2 = qz - t3[c];
+            const float ri = rsqrt32(__builtin_fmaf(d2, d2, r2ab));
+            const float ri2 = ri * ri;
+            const float q0 = m[0] * d0 + m[1] * d1 + m[2] * d2;
+            const float q1 = m[3] * d0 + m[4] * d1 + m[5] * d2;
+            const float q2 = m[6] * d0 + m[7] * d1 + m[8] * d2;
+            const float wnd = m[9] * d0 + m[10] * d1 + m[11] * d2;
+            const float gd = __builtin_fmaf(g0z, d2, gab);
+            const float Q = __builtin_fmaf(-gd, wnd, q0 * d0 + q1 * d1 + q2 * d2);
+            const float kern = Q * (ri2 * ri2 * ri);
+            v0 += d0 * kern;
+            v1 += d1 * kern;
+            v2 += d2 * kern;
+                      }
+          u0 += (double)v0; u1 += (double)v1; u2 += (double)v2;
         }
       }
     }
   } else {
-    const double g0x = A.tsub2[i], g0y = A.tsub2[A.ntp + i], g0z = A.tsub2[2 * A.ntp + i];
-    const double g0[3] = {g0x, g0y, g0z};
-    int kk = 0;
-    for (int a = 0; a < np1; a++) {
-      double d0 = px - t1[a], ti = d0 * d0;
-      for (int b = 0; b < np1; b++) {
-        double d1 = py - t2[b], tj = d1 * d1;
-        for (int c = 0; c < np1; c++, kk++) {
-          double d2 = pz - t3[c];
-          double wn0 = M[9 * pflat + kk], wn1 = M[10 * pflat + kk], wn2 = M[11 * pflat + kk];
-          double tmp[3];
-#pragma unroll
-          for (int l = 0; l < 3; l++) {
-            double a1 = M[(3 * l) * pflat + kk] - g0[l] * wn0;
-            double a2 = M[(3 * l + 1) * pflat + kk] - g0[l] * wn1;
-            double a3 = M[(3 * l + 2) * pflat + kk] - g0[l] * wn2;
-            tmp[l] = a1 * d0 + a2 * d1 + a3 * d2;
+    const double* t1 = A.cheb + (size_t)3 * np1 * node;
+    const double* t2 = t1 + np1;
+    const double* t3 = t2 + np1;
+    constexpr int NC = DL ? 12 : 6;
+    const double* M = A.mom + (size_t)fi * pflat * NC;
+    if (!DL) {
+      const double sub = A.tsub2[i];
+      for (int a = 0, kk = 0; a < np1; a++) {
+        const double d0 = px - t1[a];
+        for (int b = 0; b < np1; b++) {
+          const double d1 = py - t2[b];
+          const double r2ab = d0 * d0 + d1 * d1;
+          for (int c = 0; c < np1; c++, kk++) {
+            const double* m = M + (size_t)kk * NC;
+            const double d2 = pz - t3[c];
+            const double ri = rsqrt64(__builtin_fma(d2, d2, r2ab));
+            const double m0 = m[0] - sub * m[3], m1 = m[1] - sub * m[4], m2 = m[2] - sub * m[5];
+            const double tq = (m0 * d0 + m1 * d1 + m2 * d2) * (ri * ri);
+            u0 += (m0 + tq * d0) * ri;
+            u1 += (m1 + tq * d1) * ri;
+            u2 += (m2 + tq * d2) * ri;
           }
-          if (MIXED) {
-            float fd0 = (float)d0, fd1 = (float)d1, fd2 = (float)d2;
-            float r2 = fd0 * fd0 + fd1 * fd1 + fd2 * fd2;
-            float ri = rsqrt32(r2), ri2 = ri * ri;
-            float kern = ((float)tmp[0] * fd0 + (float)tmp[1] * fd1 + (float)tmp[2] * fd2) * (ri2 * ri2 * ri);
-            u0 += (double)(fd0 * kern);
-            u1 += (double)(fd1 * kern);
-            u2 += (double)(fd2 * kern);
-          } else {
-            double r2 = ti + tj + d2 * d2;
-            double ri = rsqrt64(r2), ri2 = ri * ri;
-            double kern = (tmp[0] * d0 + tmp[1] * d1 + tmp[2] * d2) * (ri2 * ri2 * ri);
+        }
+      }
+    } else {
+      const double g0x = A.tsub2[i], g0y = A.tsub2[A.ntp + i], g0z = A.tsub2[2 * A.ntp + i];
+      for (int a = 0, kk = 0; a < np1; a++) {
+        const double d0 = px - t1[a];
+        for (int b = 0; b < np1; b++) {
+          const double d1 = py - t2[b];
+          const double r2ab = d0 * d0 + d1 * d1;
+          const double gab = g0x * d0 + g0y * d1;
+          for (int c = 0; c < np1; c++, kk++) {
+            const double* m = M + (size_t)kk * NC;
+            const double d2 = pz - t3[c];
+            const double ri = rsqrt64(__builtin_fma(d2, d2, r2ab));
+            const double ri2 = ri * ri;
+            const double q0 = m[0] * d0 + m[1] * d1 + m[2] * d2;
+            const double q1 = m[3] * d0 + m[4] * d1 + m[5] * d2;
+            const double q2 = m[6] * d0 + m[7] * d1 + m[8] * d2;
+            const double wnd = m[9] * d0 + m[10] * d1 + m[11] * d2;
+            const double gd = __builtin_fma(g0z, d2, gab);
+            const double Q = __builtin_fma(-gd, wnd, q0 * d0 + q1 * d1 + q2 * d2);
+            const double kern = Q * (ri2 * ri2 * ri);
             u0 += d0 * kern;
             u1 += d1 * kern;
             u2 += d2 * kern;
@@ -486,6 +518,22 @@ int tree_plan_create(hor_plan* P, const double* sx, const double* sn, const doub
   HOR_TRY(TP.leaf_box.upload(lbox, s));
   HOR_TRY(TP.far_idx.upload(far_idx, s));
   HOR_TRY(TP.cheb.upload(T.cheb, s));
+  {
+    // node centres (midpoints of the Chebyshev-Lobatto intervals) and the
+    // node-centred FP32 points of the mixed-precision far field
+    const int np1 = TP.np1;
+    std::vector<double> nctr(3 * (size_t)nn);
+    std::vector<float> c32(T.cheb.size());
+    for (int q = 0; q < nn; q++)
+      for (int d = 0; d < 3; d++) {
+        const double* t = &T.cheb[(3 * (size_t)q + d) * np1];
+        const double cm = 0.5 * (t[0] + t[np1 - 1]);
+        nctr[3 * (size_t)q + d] = cm;
+        for (int j = 0; j < np1; j++) c32[(3 * (size_t)q + d) * np1 + j] = (float)(t[j] - cm);
+      }
+    HOR_TRY(TP.node_ctr.upload(nctr, s));
+    HOR_TRY(TP.cheb32.upload(c32, s));
+  }
   HOR_TRY(TP.near_item_node.upload(ni_node, s));
   HOR_TRY(TP.near_item_off.upload(ni_off, s));
   HOR_TRY(TP.near_item_cnt.upload(ni_cnt, s));
@@ -505,6 +553,8 @@ int tree_plan_create(hor_plan* P, const double* sx, const double* sn, const doub
   HOR_TRY(TP.fac.upload(fac, s));
   HOR_TRY(TP.mom_part.alloc((size_t)std::max(TP.n_mom_items, 1) * TP.ncomp * TP.pflat));
   HOR_TRY(TP.mom.alloc((size_t)std::max(TP.nfar_nodes, 1) * TP.ncomp * TP.pflat));
+  if (P->precision == HOR_MIXED)
+    HOR_TRY(TP.mom32.alloc((size_t)std::max(TP.nfar_nodes, 1) * (dl ? kNc32Dl : kNc32Sl) * TP.pflat));
   HOR_TRY(TP.slot_start.upload(TP.L.start, s));
   HOR_TRY(TP.slots.alloc(3 * (size_t)std::max(TP.nslot, 1)));
   // xi of the treecode off-surface double layer (set by setup_targets for
@@ -526,7 +576,9 @@ int tree_plan_eval(hor_plan* P, const double* dens_dev, const double* d0_dev, do
   const bool mixed = P->precision == HOR_MIXED;
   const bool dl = (P->kind == HOR_DL_ON || P->kind == HOR_DL_OFF);
   // density in tree order; per-target subtraction data (leaf and proxy variants)
-  HOR_TRY(launch_gather3(dens_dev, P->sperm.p, P->ns, P->nsp, P->sd.p, s));
+  HOR_TRY(launch_gather3(dens_dev, P->sperm.p, P->ns, P->nsp, P->sx.p, P->sn.p, P->sa.p, P->sd.p, P->rec.p,
+                         mixed ? P->rec32.p : nullptr,
+                         P->kind == HOR_SL_ON || P->kind == HOR_SL_OFF, s));
   HOR_TRY(prepare_targets(P, dens_dev, d0_dev));
   if (P->kind == HOR_SL_OFF)
     HOR_TRY(launch_target_f0n0_all(P->nt, P->ntp, P->tperm.p, d0_dev, P->n0_orig.p, P->tsub2.p, s));
@@ -547,7 +599,8 @@ int tree_plan_eval(hor_plan* P, const double* dens_dev, const double* d0_dev, do
     if (dl) hipLaunchKernelGGL((k_moments<true>), grid, dim3(kBlock), 0, s, M);
     else hipLaunchKernelGGL((k_moments<false>), grid, dim3(kBlock), 0, s, M);
     hipLaunchKernelGGL(k_moments_reduce, dim3(TP.nfar_nodes), dim3(kBlock), 0, s, TP.nfar_nodes, TP.ncomp,
-                       TP.pflat, TP.mom_node_first.p, TP.mom_node_nitems.p, TP.mom_part.p, TP.mom.p);
+                       TP.pflat, TP.mom_node_first.p, TP.mom_node_nitems.p, TP.mom_part.p, TP.mom.p,
+                       mixed ? TP.mom32.p : nullptr, dl ? kNc32Dl : kNc32Sl);
     HOR_HIP_TRY(hipGetLastError());
   }
   // 2. near leaves
@@ -574,6 +627,7 @@ int tree_plan_eval(hor_plan* P, const double* dens_dev, const double* d0_dev, do
     F.item_node = TP.far_item_node.p; F.item_off = TP.far_item_off.p; F.item_cnt = TP.far_item_cnt.p;
     F.tgt = TP.far_tgt.p; F.slot = TP.far_slot.p; F.far_idx = TP.far_idx.p;
     F.cheb = TP.cheb.p; F.mom = TP.mom.p; F.tx = P->tx.p; F.tsub2 = P->tsub2.p; F.slots = TP.slots.p;
+    F.node_ctr = TP.node_ctr.p; F.cheb32 = TP.cheb32.p; F.mom32 = TP.mom32.p;
     dim3 grid(TP.n_far_items);
     if (dl) {
       if (mixed) hipLaunchKernelGGL((k_far<true, true>), grid, dim3(kBlock), 0, s, F);

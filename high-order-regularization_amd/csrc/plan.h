@@ -17,10 +17,12 @@ struct hor_plan {
   bool all_near = false, on_at = false;  // HOR_ALL_NEAR / HOR_ON_AT
   double DEL = 0;
   int ns = 0, nsp = 0, nt = 0, ntp = 0;
-  int ntile = 0, tiles_per_chunk = 0, nchunk = 0;
+  int nsub = 0, subs_per_chunk = 0, nchunk = 0;
   std::vector<int> sperm_h, tperm_h;
   // sources, sorted SoA (stride nsp)
   hor::DBuf<double> sx, sn, sa, sd, sub_box, sn_orig;
+  hor::DBuf<hor::SrcRec> rec;
+  hor::DBuf<hor::SrcRec32> rec32;
   hor::DBuf<int> sperm;
   // targets, sorted SoA (stride ntp)
   hor::DBuf<int> tperm, tnear;

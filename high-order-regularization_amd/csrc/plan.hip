@@ -254,6 +254,8 @@ int setup_sources(hor_plan* P, const std::vector<int>& perm, const double* sx, c
   HOR_TRY(P->sa.upload(A, s));
   HOR_TRY(P->sub_box.upload(SB, s));
   HOR_TRY(P->sd.alloc(3 * (size_t)nsp));
+  HOR_TRY(P->rec.alloc(nsp));
+  if (P->precision == HOR_MIXED) HOR_TRY(P->rec32.alloc(nsp));
   HOR_TRY(P->sn_orig.upload(sn, 3 * (size_t)ns, s));
   HOR_TRY(P->in_dens.alloc(3 * (size_t)ns));
   return HOR_OK;
@@ -264,11 +266,13 @@ DirectArgs direct_args(hor_plan* P) {
   DirectArgs A;
   std::memset(&A, 0, sizeof(A));
   A.ns = P->ns; A.nsp = P->nsp; A.nt = P->nt; A.ntp = P->ntp;
-  A.ntile = P->ntile; A.tiles_per_chunk = P->tiles_per_chunk;
+  A.nsub = P->nsub; A.subs_per_chunk = P->subs_per_chunk;
   A.DEL = P->DEL;
   double fd = 8.0 * P->DEL * (1.0 + 1e-10);
   A.far_d2 = fd * fd;
+  A.near_r2 = 64.0 * P->DEL * P->DEL * (1.0 + 1e-12);
   A.sx = P->sx.p; A.sn = P->sn.p; A.sa = P->sa.p; A.sd = P->sd.p; A.sub_box = P->sub_box.p;
+  A.rec = P->rec.p; A.rec32 = P->precision == HOR_MIXED ? P->rec32.p : nullptr;
   A.tx = P->tx.p; A.tcoef = P->tcoef.p; A.tsub = P->tsub.p;
   A.tx0 = P->tx0.p; A.tn0 = P->tn0.p; A.tb = P->tb.p;
   A.tnear = P->tnear.p; A.wave_box = P->wave_box.p; A.partial = P->partial.p;
@@ -297,12 +301,12 @@ static int plan_create_direct(hor_plan* P, const double* sx, const double* sn, c
                               const double* tn0) {
   HOR_TRY(setup_sources(P, spatial_order(P->ns, sx, nullptr), sx, sn, sa));
   HOR_TRY(setup_targets(P, tx, tb, tx0, tn0, sx));
-  P->ntile = (P->ns + kBlock - 1) / kBlock;
+  P->nsub = (P->ns + kWave - 1) / kWave;
   int nblk = P->ntp / kBlock;
   int want = std::max(1, (2048 + nblk - 1) / nblk);  // >= 8 workgroups per CU
-  int nchunk = std::max(1, std::min(P->ntile, want));
-  P->tiles_per_chunk = (P->ntile + nchunk - 1) / nchunk;
-  P->nchunk = (P->ntile + P->tiles_per_chunk - 1) / P->tiles_per_chunk;
+  int nchunk = std::max(1, std::min(P->nsub, want));
+  P->subs_per_chunk = (P->nsub + nchunk - 1) / nchunk;
+  P->nchunk = (P->nsub + P->subs_per_chunk - 1) / P->subs_per_chunk;
   HOR_TRY(P->partial.alloc((size_t)P->nchunk * 3 * P->ntp));
   P->direct_pairs = (double)P->ns * (double)P->nt;
   return HOR_OK;
@@ -312,7 +316,9 @@ static int plan_eval_direct(hor_plan* P, const double* dens_dev, const double* d
                             const double* base, double coef, int accumulate) {
   hor_ctx* c = P->ctx;
   hipStream_t s = c->stream;
-  HOR_TRY(launch_gather3(dens_dev, P->sperm.p, P->ns, P->nsp, P->sd.p, s));
+  HOR_TRY(launch_gather3(dens_dev, P->sperm.p, P->ns, P->nsp, P->sx.p, P->sn.p, P->sa.p, P->sd.p, P->rec.p,
+                         P->precision == HOR_MIXED ? P->rec32.p : nullptr,
+                         P->kind == HOR_SL_ON || P->kind == HOR_SL_OFF, s));
   HOR_TRY(prepare_targets(P, dens_dev, d0_dev));
   DirectArgs A = direct_args(P);
   HOR_HIP_TRY(hipEventRecord(c->ev[2], s));
