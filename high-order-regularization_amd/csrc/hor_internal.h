@@ -72,4 +72,8 @@ struct TargetLists {
 };
 void tree_lists(const Tree& T, int nt, const double* tx, TargetLists& L);
 
+// ---- k nearest neighbours (knn.cpp): nbr[i*k + a], sorted by distance,
+// the point itself excluded.
+int knn_graph(int n, const double* x, int k, std::vector<int>& nbr);
+
 }  // namespace hor

@@ -168,9 +168,12 @@ struct hor_solver {
   hor_plan* off[2] = {nullptr, nullptr};   // off[s]: sources of surface s at the other surface's points
   hor_plan* sl_on[2] = {nullptr, nullptr};
   hor_plan* sl_off[2] = {nullptr, nullptr};
+  // mixed precision: FP64 operator for the true residual (built on first use)
+  hor_plan* on64[2] = {nullptr, nullptr};
+  hor_plan* off64[2] = {nullptr, nullptr};
   DBuf<int> nrst[2], st[2];
   DBuf<double> co[2], g0[2];
-  DBuf<double> vin, vout, V, w, part, bvec, xvec, r;
+  DBuf<double> vin, vout, V, Z, w, part, bvec, xvec, r;
   std::vector<double> part_h;
   ~hor_solver() {
     for (int k = 0; k < 2; k++) {
@@ -178,6 +181,8 @@ struct hor_solver {
       hor_plan_destroy(off[k]);
       hor_plan_destroy(sl_on[k]);
       hor_plan_destroy(sl_off[k]);
+      hor_plan_destroy(on64[k]);
+      hor_plan_destroy(off64[k]);
     }
   }
 };
@@ -196,8 +201,9 @@ static int dot_dev(hor_solver* S, const double* x, const double* y, double* out)
 
 static inline dim3 grid_for(int n) { return dim3((n + kBlock - 1) / kBlock); }
 
-// A(u) on device pointers (main.cpp:541-677).
-static int matvec_dev(hor_solver* S, const double* u, double* Au) {
+// A(u) on device pointers (main.cpp:541-677) with the given DL plans.
+static int matvec_plans(hor_solver* S, hor_plan* const on[2], hor_plan* const off[2], const double* u,
+                        double* Au) {
   hipStream_t s = S->ctx->stream;
   const double factor = 2.0 / 3.0;
   const double* u1 = u;
@@ -210,11 +216,36 @@ static int matvec_dev(hor_solver* S, const double* u, double* Au) {
   hipLaunchKernelGGL(k_interp, grid_for(S->n1), dim3(kBlock), 0, s, S->n1, S->nrst[0].p, S->st[0].p,
                      S->co[0].p, u2, S->g0[0].p);
   HOR_HIP_TRY(hipGetLastError());
-  HOR_TRY(plan_eval_into(S->on[0], u1, nullptr, A1, u1, -factor, 0));        // u1 - 2/3 DL11
-  HOR_TRY(plan_eval_into(S->off[0], u1, S->g0[1].p, A2, u2, -factor, 0));    // u2 - 2/3 DL12
-  HOR_TRY(plan_eval_into(S->off[1], u2, S->g0[0].p, A1, nullptr, -factor, 1));  // - 2/3 DL21
-  HOR_TRY(plan_eval_into(S->on[1], u2, nullptr, A2, nullptr, -factor, 1));   // - 2/3 DL22
+  HOR_TRY(plan_eval_into(on[0], u1, nullptr, A1, u1, -factor, 0));           // u1 - 2/3 DL11
+  HOR_TRY(plan_eval_into(off[0], u1, S->g0[1].p, A2, u2, -factor, 0));       // u2 - 2/3 DL12
+  HOR_TRY(plan_eval_into(off[1], u2, S->g0[0].p, A1, nullptr, -factor, 1));  // - 2/3 DL21
+  HOR_TRY(plan_eval_into(on[1], u2, nullptr, A2, nullptr, -factor, 1));      // - 2/3 DL22
   return HOR_OK;
+}
+
+// The solver's operator (its own precision).
+static int matvec_dev(hor_solver* S, const double* u, double* Au) {
+  return matvec_plans(S, S->on, S->off, u, Au);
+}
+
+// The FP64 operator: the solver's own plans, or for a mixed-precision solver
+// a second set of FP64 plans built on first use.
+static int matvec64_dev(hor_solver* S, const double* u, double* Au) {
+  if (S->precision != HOR_MIXED) return matvec_plans(S, S->on, S->off, u, Au);
+  if (!S->on64[0]) {
+    hor_ctx* c = S->ctx;
+    for (int k = 0; k < 2; k++) {
+      const int o = 1 - k;
+      const int nk = (int)S->area[k].size(), no = (int)S->area[o].size();
+      HOR_TRY(hor_plan_create(c, HOR_DL_ON, S->method, HOR_FP64, S->DEL, nk, S->x[k].data(), S->nrml[k].data(),
+                              S->area[k].data(), nk, S->x[k].data(), nullptr, nullptr, nullptr, S->ctr[k],
+                              &S->tp[k], &S->on64[k]));
+      HOR_TRY(hor_plan_create(c, HOR_DL_OFF, S->method, HOR_FP64, S->DEL, nk, S->x[k].data(),
+                              S->nrml[k].data(), S->area[k].data(), no, S->x[o].data(), S->b[o].data(),
+                              S->x0[o].data(), S->n0[o].data(), S->ctr[k], &S->tp[k], &S->off64[k]));
+    }
+  }
+  return matvec_plans(S, S->on64, S->off64, u, Au);
 }
 
 extern "C" int hor_solver_create(hor_ctx* ctx, double h, double DEL, int method, int precision,
@@ -374,54 +405,79 @@ extern "C" int hor_solver_eval_sl(hor_solver* S, const double* f1, const double*
   return HOR_OK;
 }
 
-// Restarted GMRES(m) with MGS Arnoldi and Givens rotations (main.cpp:210-416);
-// restarts from the true residual b - A x.
-extern "C" int hor_solver_gmres(hor_solver* S, const double* b, double* x, int m, double tol, int maxit,
-                                int* iters, double* relres, double* hist) {
-  if (!S || !b || !x || m < 1 || maxit < 0) return set_error(HOR_E_ARG, "hor_solver_gmres: bad argument");
+// Restarted flexible GMRES(m) with MGS Arnoldi and Givens rotations
+// (main.cpp:210-416), device-resident.  Arnoldi applies A_arn to z_j = M(v_j)
+// (z_j = v_j without a preconditioner); each cycle restarts from the true
+// residual b - A_true x (the reference restarts from b, main.cpp:286).  With
+// recheck, a cycle that meets tol by its estimate is confirmed with A_true and
+// continues if the true residual is above tol; *relres is then the true
+// residual.  Without it the solve stops on the estimate, as the reference does.
+// Optionally logs the pairs (z_j, A z_j) into device buffers.
+struct PrecondOp {
+  hor_mpnn* net = nullptr;
+};
+
+static int apply_precond(hor_solver* S, const PrecondOp& M, const double* v, double* z);
+
+static int gmres_core(hor_solver* S, bool flexible, const PrecondOp& M, bool mixed_arnoldi, bool recheck,
+                      const double* b_host, double* x_host, int m, double tol, int maxit, int* iters,
+                      double* relres, double* hist, double* log_z, double* log_az, int log_max, int* n_logged) {
   hor_ctx* c = S->ctx;
   hipStream_t s = c->stream;
   HOR_HIP_TRY(hipSetDevice(c->device));
   const int n = S->n;
   HOR_TRY(S->V.alloc((size_t)n * (m + 1)));
+  if (flexible) HOR_TRY(S->Z.alloc((size_t)n * m));
   HOR_TRY(S->w.alloc(n));
   HOR_TRY(S->bvec.alloc(n));
   HOR_TRY(S->xvec.alloc(n));
   HOR_TRY(S->r.alloc(n));
-  HOR_HIP_TRY(hipMemcpyAsync(S->bvec.p, b, sizeof(double) * n, hipMemcpyHostToDevice, s));
+  HOR_HIP_TRY(hipMemcpyAsync(S->bvec.p, b_host, sizeof(double) * n, hipMemcpyHostToDevice, s));
   HOR_HIP_TRY(hipMemsetAsync(S->xvec.p, 0, sizeof(double) * n, s));
+  auto A_arn = [&](const double* u, double* Au) { return mixed_arnoldi ? matvec_dev(S, u, Au) : matvec64_dev(S, u, Au); };
   std::vector<double> H((size_t)(m + 1) * m, 0.0), g(m + 1, 0.0), y(m + 1, 0.0), cs(m, 0.0), sn(m, 0.0);
   auto Hh = [&](int i, int j) -> double& { return H[(size_t)i * m + j]; };
   auto Vp = [&](int i) { return S->V.p + (size_t)i * n; };
+  auto Zp = [&](int i) { return flexible ? S->Z.p + (size_t)i * n : S->V.p + (size_t)i * n; };
   dim3 gr = grid_for(n), bl(kBlock);
   double bnorm2;
   HOR_TRY(dot_dev(S, S->bvec.p, S->bvec.p, &bnorm2));
-  double b_norm = std::sqrt(bnorm2);
-  int iter = 0;
+  const double b_norm = std::sqrt(bnorm2);
+  int iter = 0, logged = 0;
   double error = 1.0;
+  bool have_x = false;
   while (true) {
-    if (error <= tol || iter >= maxit) break;
-    // r = b - A x (x = 0 on the first cycle: r = b)
-    if (iter > 0) {
-      HOR_TRY(matvec_dev(S, S->xvec.p, S->r.p));
+    // r = b - A_true x (x = 0 on the first cycle: r = b)
+    if (have_x) {
+      if (iter >= maxit) break;
+      HOR_TRY(matvec64_dev(S, S->xvec.p, S->r.p));
       hipLaunchKernelGGL(k_sub, gr, bl, 0, s, n, S->bvec.p, S->r.p, Vp(0));
     } else {
       HOR_HIP_TRY(hipMemcpyAsync(Vp(0), S->bvec.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
     }
     double rn2;
     HOR_TRY(dot_dev(S, Vp(0), Vp(0), &rn2));
-    double r_norm = std::sqrt(rn2);
+    const double r_norm = std::sqrt(rn2);
     error = r_norm;
     if (b_norm > 1.e-8) error /= b_norm;
-    if (error <= tol || r_norm == 0.0) break;
+    if (error <= tol || r_norm == 0.0 || iter >= maxit) break;
     hipLaunchKernelGGL(k_scale, gr, bl, 0, s, n, 1.0 / r_norm, Vp(0), Vp(0));
     std::fill(g.begin(), g.end(), 0.0);
     g[0] = r_norm;
     int j = -1;
-    while ((j < m - 1) && (error > tol) && (iter < maxit)) {
+    double est = error;
+    while ((j < m - 1) && (est > tol) && (iter < maxit)) {
       j = j + 1;
       iter = iter + 1;
-      HOR_TRY(matvec_dev(S, Vp(j), Vp(j + 1)));
+      if (flexible) HOR_TRY(apply_precond(S, M, Vp(j), Zp(j)));
+      HOR_TRY(A_arn(Zp(j), Vp(j + 1)));
+      if (log_z && logged < log_max) {
+        HOR_HIP_TRY(hipMemcpyAsync(log_z + (size_t)logged * n, Zp(j), sizeof(double) * n,
+                                   hipMemcpyDeviceToDevice, s));
+        HOR_HIP_TRY(hipMemcpyAsync(log_az + (size_t)logged * n, Vp(j + 1), sizeof(double) * n,
+                                   hipMemcpyDeviceToDevice, s));
+        logged++;
+      }
       for (int i = 0; i <= j; i++) {  // modified Gram-Schmidt
         double t;
         HOR_TRY(dot_dev(S, Vp(i), Vp(j + 1), &t));
@@ -446,11 +502,11 @@ extern "C" int hor_solver_gmres(hor_solver* S, const double* b, double* x, int m
       Hh(j + 1, j) = 0.0;
       g[j + 1] = -sn[j] * g[j];
       g[j] = cs[j] * g[j];
-      error = std::fabs(g[j + 1]);
-      if (b_norm > 1.e-8) error /= b_norm;
-      if (hist) hist[iter - 1] = error;
+      est = std::fabs(g[j + 1]);
+      if (b_norm > 1.e-8) est /= b_norm;
+      if (hist) hist[iter - 1] = est;
     }
-    // x += V y with H y = g (upper triangular)
+    // x += Z y with H y = g (upper triangular)
     for (int i = 0; i <= j; i++) y[i] = g[i];
     y[j] = y[j] / Hh(j, j);
     for (int k = j - 1; k >= 0; k--) {
@@ -458,12 +514,49 @@ extern "C" int hor_solver_gmres(hor_solver* S, const double* b, double* x, int m
       for (int i = k + 1; i <= j; i++) t -= Hh(k, i) * y[i];
       y[k] = t / Hh(k, k);
     }
-    for (int i = 0; i <= j; i++) hipLaunchKernelGGL(k_axpy, gr, bl, 0, s, n, y[i], Vp(i), S->xvec.p);
+    for (int i = 0; i <= j; i++) hipLaunchKernelGGL(k_axpy, gr, bl, 0, s, n, y[i], Zp(i), S->xvec.p);
     HOR_HIP_TRY(hipGetLastError());
+    have_x = true;
+    error = est;
+    if (!recheck && est <= tol) break;
   }
-  HOR_HIP_TRY(hipMemcpyAsync(x, S->xvec.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+  HOR_HIP_TRY(hipMemcpyAsync(x_host, S->xvec.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
   HOR_HIP_TRY(hipStreamSynchronize(s));
   if (iters) *iters = iter;
   if (relres) *relres = error;
+  if (n_logged) *n_logged = logged;
   return HOR_OK;
+}
+
+// GMRES(m) of test_Integral_Equation_gmres.  FP64: stops on the residual
+// estimate as the reference does.  Mixed precision: Arnoldi with the mixed
+// operator, restarts and the final check with the FP64 operator.
+extern "C" int hor_solver_gmres(hor_solver* S, const double* b, double* x, int m, double tol, int maxit,
+                                int* iters, double* relres, double* hist) {
+  if (!S || !b || !x || m < 1 || maxit < 0) return set_error(HOR_E_ARG, "hor_solver_gmres: bad argument");
+  const bool mixed = S->precision == HOR_MIXED;
+  return gmres_core(S, false, PrecondOp{}, mixed, mixed, b, x, m, tol, maxit, iters, relres, hist, nullptr,
+                    nullptr, 0, nullptr);
+}
+
+// Flexible GMRES(m) with an optional right preconditioner (a trained
+// point-cloud network); the final residual is always the FP64 true residual.
+extern "C" int hor_solver_fgmres(hor_solver* S, const double* b, double* x, int m, double tol, int maxit,
+                                 hor_mpnn* precond, int* iters, double* relres, double* hist, double* log_z_dev,
+                                 double* log_az_dev, int log_max, int* n_logged) {
+  if (!S || !b || !x || m < 1 || maxit < 0 || log_max < 0 || ((log_z_dev == nullptr) != (log_az_dev == nullptr)))
+    return set_error(HOR_E_ARG, "hor_solver_fgmres: bad argument");
+  PrecondOp M;
+  M.net = precond;
+  if (precond) HOR_TRY(hor_mpnn_check_size(precond, S->n));
+  return gmres_core(S, true, M, S->precision == HOR_MIXED, true, b, x, m, tol, maxit, iters, relres, hist,
+                    log_z_dev, log_az_dev, log_max, n_logged);
+}
+
+static int apply_precond(hor_solver* S, const PrecondOp& M, const double* v, double* z) {
+  if (!M.net) {
+    HOR_HIP_TRY(hipMemcpyAsync(z, v, sizeof(double) * S->n, hipMemcpyDeviceToDevice, S->ctx->stream));
+    return HOR_OK;
+  }
+  return hor_mpnn_apply_dev(M.net, S->ctx, v, z);
 }

@@ -65,6 +65,20 @@ _SIGS = {
     "hor_solver_eval_sl": (_I, [_P, _P, _P, _P, _P, _P]),
     "hor_solver_interp_g0": (_I, [_P, _I, _P, _P]),
     "hor_solver_gmres": (_I, [_P, _P, _P, _I, _D, _I, _IP, _DP, _P]),
+    "hor_solver_fgmres": (_I, [_P, _P, _P, _I, _D, _I, _P, _IP, _DP, _P, _P, _P, _I, _IP]),
+    "hor_mpnn_param_count": (_I, []),
+    "hor_mpnn_create": (_I, [_P, _I, C.POINTER(_P)]),
+    "hor_mpnn_destroy": (_I, [_P]),
+    "hor_mpnn_set_params": (_I, [_P, _P]),
+    "hor_mpnn_get_params": (_I, [_P, _P]),
+    "hor_mpnn_get_grad": (_I, [_P, _P]),
+    "hor_mpnn_set_geometry": (_I, [_P, _I, _I, _P, _P, _P, _P, _I, _P]),
+    "hor_mpnn_check_size": (_I, [_P, _I]),
+    "hor_mpnn_forward_dev": (_I, [_P, _I, _P, _P]),
+    "hor_mpnn_apply_dev": (_I, [_P, _P, _P, _P]),
+    "hor_mpnn_loss_grad_dev": (_I, [_P, _I, _P, _P, _DP]),
+    "hor_mpnn_adam_step": (_I, [_P, C.c_float, C.c_float, C.c_float, C.c_float]),
+    "hor_knn_graph": (_I, [_I, _P, _I, _P]),
     "hor_dev_alloc": (_I, [_P, _S, C.POINTER(_P)]),
     "hor_dev_free": (_I, [_P, _P]),
     "hor_dev_upload": (_I, [_P, _P, _P, _S]),

@@ -8,6 +8,7 @@ device raises instead of falling back to the CPU.
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 
 import numpy as np
 
@@ -35,15 +36,24 @@ def TreeParams(P=6, N0=1000, theta=0.6):
 
 
 class Context:
-    """One GPU, one HIP stream."""
+    """One GPU, one HIP stream.  Objects created on the context (plans,
+    solvers, networks) are closed before the context is destroyed."""
 
     def __init__(self, device=0):
         h = C.c_void_p()
         check(_lib.lib().hor_ctx_create(int(device), C.byref(h)), "hor_ctx_create")
         self.h = h
+        self._children = weakref.WeakSet()
+
+    def adopt(self, obj):
+        """Registers obj (with a close() method) to be closed with the context."""
+        self._children.add(obj)
+        return obj
 
     def close(self):
         if self.h:
+            for obj in list(self._children):
+                obj.close()
             _lib.lib().hor_ctx_destroy(self.h)
             self.h = None
 
@@ -110,6 +120,7 @@ class Plan:
                                          ptr(k[0]), ptr(k[1]), ptr(k[2]), nt, ptr(k[3]), ptr(k[4]),
                                          ptr(k[5]), ptr(k[6]), ptr(c), tp, C.byref(h)), "hor_plan_create")
         self.h, self.ctx, self.kind, self.ns, self.nt = h, ctx, kind, ns, nt
+        ctx.adopt(self)
 
     def close(self):
         if self.h:

@@ -42,6 +42,7 @@ class Solver:
                                            C.byref(self.d1), C.byref(self.d2), C.byref(h_)),
               "hor_solver_create")
         self.h, self.ctx = h_, ctx
+        ctx.adopt(self)
         n = C.c_int()
         check(_lib.lib().hor_solver_size(self.h, C.byref(n)), "hor_solver_size")
         self.n = n.value
@@ -81,6 +82,22 @@ class Solver:
         a = [f64(f1, 3 * self.n1), f64(f2, 3 * self.n2), f64(f01, 3 * self.n1), f64(f02, 3 * self.n2)]
         check(_lib.lib().hor_solver_eval_sl(self.h, *[ptr(x) for x in a], ptr(out)), "hor_solver_eval_sl")
         return out
+
+    def fgmres(self, b, m=15, tol=1e-10, maxit=100, precond=None, log=None):
+        """Flexible GMRES(m) with an optional right preconditioner (a
+        ``hor.mpnn.Network`` bound to this geometry).  The returned residual is
+        the FP64 true residual |b - A x|/|b|.  ``log`` = (z_dev, az_dev,
+        log_max) device buffers receive the Arnoldi pairs (z_j, A z_j).
+        Returns x, iterations, residual-estimate history, residual, n_logged."""
+        b = f64(b, self.n)
+        x = np.zeros(self.n)
+        hist = np.zeros(max(maxit, 1))
+        it, res, nl = C.c_int(), C.c_double(), C.c_int()
+        lz, laz, lmax = (None, None, 0) if log is None else log
+        check(_lib.lib().hor_solver_fgmres(self.h, ptr(b), ptr(x), int(m), float(tol), int(maxit),
+                                           None if precond is None else precond.h, C.byref(it), C.byref(res),
+                                           ptr(hist), lz, laz, int(lmax), C.byref(nl)), "hor_solver_fgmres")
+        return x, it.value, hist[: it.value], res.value, nl.value
 
     def gmres(self, b, m=15, tol=1e-10, maxit=100):
         b = f64(b, self.n)

@@ -221,11 +221,65 @@ int hor_solver_interp_g0(hor_solver* solver, int k, const double* u, double* g0)
 
 /* Restarted GMRES(m) (main.cpp:210-416), device-resident, x0 = 0, restart from
  * the true residual b - Ax.  b, x host pointers of length n.  hist (may be
- * NULL) receives up to maxit relative residual estimates.  On return *iters is
- * the iteration count and *relres the final estimate |g_{j+1}|/|b|. */
+ * NULL) receives up to maxit relative residual estimates.  FP64 solver: stops
+ * on the estimate and *relres is the final estimate |g_{j+1}|/|b|, as in the
+ * reference.  Mixed-precision solver: the Arnoldi steps use the mixed operator,
+ * every restart residual and the final check use the FP64 operator, a cycle
+ * whose FP64 residual is above tol continues, and *relres is that FP64
+ * residual. */
 int hor_solver_gmres(hor_solver* solver, const double* b, double* x, int m,
                      double tol, int maxit, int* iters, double* relres,
                      double* hist);
+
+/* ------------------------------------------------------------------------
+ * Learned right preconditioner.  No counterpart in the reference (its GMRES,
+ * main.cpp:313-415, is unpreconditioned): a point-cloud message-passing
+ * network (width 64, k nearest neighbours, 3 layers, BF16 matrix-core GEMMs
+ * with FP32 accumulation) z = M(r) over the points of both surfaces, trained
+ * on the GPU from (z, A z) pairs logged by hor_solver_fgmres.
+ * ------------------------------------------------------------------------ */
+typedef struct hor_mpnn hor_mpnn;
+
+/* Number of FP32 parameters (fixed architecture). */
+int hor_mpnn_param_count(void);
+/* Network with k neighbours per point; parameters start at zero (M = I). */
+int hor_mpnn_create(hor_ctx* ctx, int k, hor_mpnn** net);
+int hor_mpnn_destroy(hor_mpnn* net);
+/* Host FP32 parameter vectors of hor_mpnn_param_count() values; setting the
+ * parameters resets the Adam state. */
+int hor_mpnn_set_params(hor_mpnn* net, const float* params);
+int hor_mpnn_get_params(hor_mpnn* net, float* params);
+int hor_mpnn_get_grad(hor_mpnn* net, float* grad);
+/* Point set in the solver's unknown order (n1 points of surface 1, then n2 of
+ * surface 2; x, nrml 3*(n1+n2)), body centres ctr[6] and sizes size[2], and
+ * the largest batch of vectors evaluated at once.  Builds the k-nearest-
+ * neighbour graph on the host; nbr_out (may be NULL) receives it,
+ * k*(n1+n2) indices. */
+int hor_mpnn_set_geometry(hor_mpnn* net, int n1, int n2, const double* x,
+                          const double* nrml, const double* ctr, const double* size,
+                          int batch_max, int* nbr_out);
+/* HOR_OK if the geometry has n/3 points. */
+int hor_mpnn_check_size(hor_mpnn* net, int n);
+/* z = M(r) for batch vectors of 3*(n1+n2) values each (device pointers). */
+int hor_mpnn_forward_dev(hor_mpnn* net, int batch, const double* r_dev, double* z_dev);
+/* z = M(r) on ctx's stream without synchronizing (the solver's call). */
+int hor_mpnn_apply_dev(hor_mpnn* net, hor_ctx* ctx, const double* r_dev, double* z_dev);
+/* Forward on r, then the loss mean_b |M(r_b) - v_b|^2/|v_b|^2 and its gradient
+ * with respect to the parameters (kept on the device). */
+int hor_mpnn_loss_grad_dev(hor_mpnn* net, int batch, const double* r_dev,
+                           const double* v_dev, double* loss);
+/* One Adam step with the last gradient. */
+int hor_mpnn_adam_step(hor_mpnn* net, float lr, float beta1, float beta2, float eps);
+
+/* Flexible GMRES(m) with the right preconditioner precond (NULL: none, the
+ * plain method).  The solve always ends on the FP64 true residual: *relres is
+ * |b - A x|/|b| with the FP64 operator.  When log_z_dev / log_az_dev are given
+ * (device buffers of log_max*n values), the first log_max Arnoldi pairs
+ * (z_j, A z_j) are copied there and *n_logged is their count. */
+int hor_solver_fgmres(hor_solver* solver, const double* b, double* x, int m,
+                      double tol, int maxit, hor_mpnn* precond, int* iters,
+                      double* relres, double* hist, double* log_z_dev,
+                      double* log_az_dev, int log_max, int* n_logged);
 
 /* ------------------------------------------------------------------------
  * Device memory helpers (for callers who keep data resident).
@@ -290,6 +344,11 @@ int hor_traction_translating(const hor_body* body, const double* U, int n,
                              const double* x, double* f);
 int hor_spheroid_exact_velocity(const hor_body* body, int n, const double* x,
                                 double* u);
+
+/* k nearest neighbours of each of n points (host): nbr[i*k + a], sorted by
+ * (distance, index), the point itself excluded; the graph of the learned
+ * preconditioner. */
+int hor_knn_graph(int n, const double* x, int k, int* nbr);
 
 #ifdef __cplusplus
 }

@@ -89,3 +89,25 @@ def test_gmres_restart_path(ctx, oracle):
     print("restart gmres", it_o, it_g, res, flush=True)
     assert abs(it_g - it_o) <= 1
     assert rel_l2(x_g, x_o) <= 1e-8
+
+
+def test_mixed_gmres_ends_on_fp64_residual(ctx):
+    """Mixed-precision treecode GMRES: Arnoldi with the FP32-far-field
+    operator, restarts and the final check with the FP64 operator, so the
+    reported residual is the FP64 one and meets the tolerance."""
+    h, DEL, bodies, surfs, tgts = sphere_pair(32, gap=0.1)
+    pd = problem_dicts(bodies, surfs, tgts)
+    tp = hor.TreeParams(6, 500, 0.6)
+    s64 = S.Solver(ctx, h, DEL, pd[0], pd[1], method=hor.TREE, precision=hor.FP64, tree1=tp, tree2=tp)
+    smx = S.Solver(ctx, h, DEL, pd[0], pd[1], method=hor.TREE, precision=hor.MIXED, tree1=tp, tree2=tp)
+    b = np.random.default_rng(4).uniform(-1, 1, s64.n)
+    for tol in (1e-8, 1e-11):
+        x, it, hist, res = smx.gmres(b, m=15, tol=tol, maxit=200)
+        true = np.linalg.norm(b - s64.matvec(x)) / np.linalg.norm(b)
+        x64, it64, _, res64 = s64.gmres(b, m=15, tol=tol, maxit=200)
+        print(f"tol {tol:g}: mixed {it} iterations, FP64 residual {res:.3e} (recomputed {true:.3e}); "
+              f"FP64 solve {it64} iterations")
+        assert res <= tol and abs(true - res) <= 1e-3 * res + 1e-16
+        assert np.linalg.norm(x - x64) <= 10 * tol * np.linalg.norm(x64) / 0.3
+    s64.close()
+    smx.close()
