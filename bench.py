@@ -166,25 +166,46 @@ def gmres_workload(h=1.0 / 128.0, gap=0.1, P=8, N0=4000, theta=0.6, tol=1e-8, se
             S.matvec(b_rigid)
         out.setdefault("matvec_ms", {})[name] = 1e3 * (time.perf_counter() - t0) / 3
     S_dir = Solver(ctx, h, DEL, pd[0], pd[1], method=hor.DIRECT, precision=hor.FP64) if direct_check else None
+    # learned right preconditioner (trained weights shipped with the package)
+    from hor import mpnn as MP
+    params = MP.load_weights()
+    net = None
+    if params is not None:
+        t0 = time.time()
+        net = MP.Network(ctx, params=params)
+        net.set_geometry(pd, bodies)
+        out["learned_setup_s"] = time.time() - t0
     for rhs_name, b in (("rigid", b_rigid), ("forcing", b_force)):
         res = {}
         xs = {}
-        for name, S in solvers.items():
+        runs = [(name, S, None) for name, S in solvers.items()]
+        if net is not None:
+            runs.append(("mixed_learned", solvers["mixed"], net))
+        for name, S, M in runs:
             t0 = time.perf_counter()
-            x, it, hist, est = S.gmres(b, m=15, tol=tol, maxit=300)
+            if M is None:
+                x, it, hist, est = S.gmres(b, m=15, tol=tol, maxit=300)
+                info = {"iterations": it, "relres_estimate": est}
+            else:
+                x, st = MP.solve(S, b, net=M, m=15, tol=tol, maxit=300)
+                info = {"iterations": st["learned_iters"] + st["plain_iters"], "learned_iterations": st["learned_iters"],
+                        "fallback_plain_iterations": st["plain_iters"], "relres_estimate": st["relres"]}
             dt = time.perf_counter() - t0
             xs[name] = x
             r = b - S64.matvec(x)
-            res[name] = {"iterations": it, "seconds": dt, "ms_per_iteration": 1e3 * dt / max(it, 1),
-                         "relres_estimate": est,
-                         "relres_fp64_recheck": float(np.linalg.norm(r) / np.linalg.norm(b))}
+            res[name] = dict(info, seconds=dt, ms_per_iteration=1e3 * dt / max(info["iterations"], 1),
+                             relres_fp64_recheck=float(np.linalg.norm(r) / np.linalg.norm(b)))
             if S_dir is not None:
                 rd = b - S_dir.matvec(x)
                 res[name]["relres_fp64_direct"] = float(np.linalg.norm(rd) / np.linalg.norm(b))
-        res["mixed"]["rel_diff_vs_fp64_solution"] = float(np.linalg.norm(xs["mixed"] - xs["fp64"]) /
-                                                          np.linalg.norm(xs["fp64"]))
-        res["speedup_mixed_vs_fp64"] = res["fp64"]["seconds"] / res["mixed"]["seconds"]
+        for name in xs:
+            if name != "fp64":
+                res[name]["rel_diff_vs_fp64_solution"] = float(np.linalg.norm(xs[name] - xs["fp64"]) /
+                                                               np.linalg.norm(xs["fp64"]))
+                res[f"speedup_{name}_vs_fp64"] = res["fp64"]["seconds"] / res[name]["seconds"]
         out[rhs_name] = res
+    if net is not None:
+        net.close()
     for S in solvers.values():
         S.close()
     if S_dir is not None:

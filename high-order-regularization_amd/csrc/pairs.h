@@ -66,7 +66,7 @@ __device__ __forceinline__ void sl_plain(const SrcRec& s, const SrcRec32& s32, d
 // from one reciprocal square root; the self test r < 1e-14 is r2 < 1e-28.
 __device__ __forceinline__ void sl_full(const SrcRec& s, double px, double py, double pz,
                                         double sub, bool nearp, const double* q, double inv_del,
-                                        double self1, double self2, double u[3]) {
+                                        double self1, double self2, const double* ec, double u[3]) {
   double dx = px - s.x, dy = py - s.y, dz = pz - s.z;
   double r2 = dx * dx + dy * dy + dz * dz;
   double fx = s.d0 - sub * s.nx, fy = s.d1 - sub * s.ny, fz = s.d2 - sub * s.nz;
@@ -84,8 +84,9 @@ __device__ __forceinline__ void sl_full(const SrcRec& s, double px, double py, d
       s2 = ri3;
     } else {
       double rd2 = rd * rd;
-      double ef = erf(rd);
-      double rex = rd * exp(-rd2);
+      double E = exp(-rd2);
+      double ef = erf_from_exp(rd, E, ec);
+      double rex = rd * E;
       double p1 = q[0] + rd2 * (q[1] + rd2 * q[2]);
       double p2 = q[3] + rd2 * (q[4] + rd2 * (q[5] + rd2 * q[6]));
       s1 = (ef + p1 * rex) * ri;
@@ -135,7 +136,7 @@ __device__ __forceinline__ void dl_plain(const SrcRec& s, const SrcRec32& s32, d
 // Every term is linear in the source normal, so the record's a n carries the
 // weight.
 __device__ __forceinline__ void dl_off_full(const SrcRec& s, const Tgt& t, double inv_del,
-                                            double self2, double u[3]) {
+                                            double self2, const double* ec, double u[3]) {
   double dx = t.px - s.x, dy = t.py - s.y, dz = t.pz - s.z;
   double r2 = dx * dx + dy * dy + dz * dz;
   double gx = s.d0 - t.g0x, gy = s.d1 - t.g0y, gz = s.d2 - t.g0z;
@@ -158,8 +159,9 @@ __device__ __forceinline__ void dl_off_full(const SrcRec& s, const Tgt& t, doubl
     }
     double ri3 = ri2 * ri;
     double rd2 = rd * rd;
-    double ef = erf(rd);
-    double rex = rd * exp(-rd2);
+    double E = exp(-rd2);
+    double ef = erf_from_exp(rd, E, ec);
+    double rex = rd * E;
     const double* q = t.q;
     double p2 = q[3] + rd2 * (q[4] + rd2 * (q[5] + rd2 * q[6]));
     double p3 = q[7] + rd2 * (q[8] + rd2 * (q[9] + rd2 * (q[10] + rd2 * q[11])));
@@ -186,7 +188,7 @@ __device__ __forceinline__ void dl_off_full(const SrcRec& s, const Tgt& t, doubl
 
 // ---- DL on-surface pair, full branch structure (stresslets.cpp:244-269).
 __device__ __forceinline__ void dl_on_full(const SrcRec& s, const Tgt& t, double inv_del,
-                                           double u[3]) {
+                                           const double* ec, double u[3]) {
   double dx = t.px - s.x, dy = t.py - s.y, dz = t.pz - s.z;
   double r2 = dx * dx + dy * dy + dz * dz;
   if (r2 < 1e-28) return;  // d = 0: the self term contributes exactly zero
@@ -198,8 +200,9 @@ __device__ __forceinline__ void dl_on_full(const SrcRec& s, const Tgt& t, double
     H3 = ri2 * ri2 * ri;
   } else {
     double rd2 = rd * rd;
-    double ef = erf(rd);
-    double rex = rd * exp(-rd2);
+    double E = exp(-rd2);
+    double ef = erf_from_exp(rd, E, ec);
+    double rex = rd * E;
     const double* q = t.q;
     double p3 = q[7] + rd2 * (q[8] + rd2 * (q[9] + rd2 * q[10]));
     H3 = (ef + p3 * rex) * (ri2 * ri2 * ri);
@@ -274,11 +277,11 @@ __device__ __forceinline__ void range_sum(const DirectArgs& A, int j0, int j1, i
         const bool need = t.nearp && !(r2 > A.near_r2);
         if (__any(need ? 1 : 0)) {
           if (KIND == HOR_SL_ON || KIND == HOR_SL_OFF)
-            sl_full(s, t.px, t.py, t.pz, t.sub, t.nearp, t.q, inv_del, self1, self2, u);
+            sl_full(s, t.px, t.py, t.pz, t.sub, t.nearp, t.q, inv_del, self1, self2, A.erfcx, u);
           else if (KIND == HOR_DL_OFF)
-            dl_off_full(s, t, inv_del, self2, u);
+            dl_off_full(s, t, inv_del, self2, A.erfcx, u);
           else
-            dl_on_full(s, t, inv_del, u);
+            dl_on_full(s, t, inv_del, A.erfcx, u);
         } else {
           if (KIND == HOR_SL_ON || KIND == HOR_SL_OFF)
             sl_plain<false>(s, SrcRec32{}, t.px, t.py, t.pz, pxf, pyf, pzf, t.sub, u);

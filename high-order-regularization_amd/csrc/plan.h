@@ -23,6 +23,7 @@ struct hor_plan {
   hor::DBuf<double> sx, sn, sa, sd, sub_box, sn_orig;
   hor::DBuf<hor::SrcRec> rec;
   hor::DBuf<hor::SrcRec32> rec32;
+  hor::DBuf<double> erfcx;   // coefficients of the erf polynomial (kErfcxC)
   hor::DBuf<int> sperm;
   // targets, sorted SoA (stride ntp)
   hor::DBuf<int> tperm, tnear;
