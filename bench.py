@@ -134,8 +134,27 @@ def two_sphere_problem(h, gap, seed=SEED):
     return DEL, bodies, pd, np.concatenate(rigid)
 
 
+def cpu_matvec(h, DEL, pd, P, N0, theta, u, reps=1):
+    """One treecode matvec of the two-surface operator by the oracle
+    restatement on all host cores (kind "port": the reference's KITC fixes
+    P = 6, N0 = 1000 at compile time, KITC.h:14-16, so it cannot run these
+    parameters; at its own parameters the restatement and the compiled
+    reference take the same time and give identical results)."""
+    from pyoracle import Oracle
+    O = Oracle()
+    pb = O.problem(h, DEL, pd, use_tree=True, tree=((P, N0, theta), (P, N0, theta)))
+    t0 = time.perf_counter()
+    for k in range(reps):
+        y = O.matvec(pb, u)
+        log(f"  CPU matvec {k + 1}/{reps}: {time.perf_counter() - t0:.1f}s")
+    t_mv = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    O.eval_sl(pb)
+    return t_mv, time.perf_counter() - t0, y
+
+
 def gmres_workload(h=1.0 / 128.0, gap=0.1, P=8, N0=4000, theta=0.6, tol=1e-8, seed=SEED,
-                   direct_check=False):
+                   direct_check=False, cpu=False, cpu_reps=1):
     """Second-kind double-layer equation A u = u - 2/3 DL u = b on two spheres
     (two_sphere_problem), treecode matvec, GMRES(15) to relative residual tol,
     FP64 and mixed-precision treecodes, both right-hand sides.  Every solution's
@@ -165,6 +184,19 @@ def gmres_workload(h=1.0 / 128.0, gap=0.1, P=8, N0=4000, theta=0.6, tol=1e-8, se
         for _ in range(3):
             S.matvec(b_rigid)
         out.setdefault("matvec_ms", {})[name] = 1e3 * (time.perf_counter() - t0) / 3
+    if cpu:
+        try:
+            cores = len(os.sched_getaffinity(0))
+        except AttributeError:
+            cores = os.cpu_count() or 1
+        log(f"CPU treecode matvec (oracle restatement, {cores} threads)")
+        t_cpu, t_rhs, y_cpu = cpu_matvec(h, DEL, pd, P, N0, theta, b_force, reps=cpu_reps)
+        y_gpu = S64.matvec(b_force)
+        out["cpu_baseline"] = {"matvec_s": t_cpu, "matvecs_timed": cpu_reps, "rhs_s": t_rhs, "cores": cores,
+                               "kind": "port",
+                               "gpu_fp64_matvec_rel_diff": float(np.linalg.norm(y_gpu - y_cpu) / np.linalg.norm(y_cpu)),
+                               "note": f"mean of {cpu_reps} timed matvec(s); CPU GMRES time = iterations x matvec "
+                                       "(extrapolated, the RHS excluded as on the GPU side)"}
     S_dir = Solver(ctx, h, DEL, pd[0], pd[1], method=hor.DIRECT, precision=hor.FP64) if direct_check else None
     # learned right preconditioner (trained weights shipped with the package)
     from hor import mpnn as MP
@@ -204,6 +236,10 @@ def gmres_workload(h=1.0 / 128.0, gap=0.1, P=8, N0=4000, theta=0.6, tol=1e-8, se
                                                                np.linalg.norm(xs["fp64"]))
                 res[f"speedup_{name}_vs_fp64"] = res["fp64"]["seconds"] / res[name]["seconds"]
         out[rhs_name] = res
+        if "cpu_baseline" in out:
+            it = res["fp64"]["iterations"]
+            out["cpu_baseline"][f"gmres_s_{rhs_name}"] = it * out["cpu_baseline"]["matvec_s"]
+            res["speedup_fp64_gpu_vs_cpu"] = it * out["cpu_baseline"]["matvec_s"] / res["fp64"]["seconds"]
     if net is not None:
         net.close()
     for S in solvers.values():
@@ -232,6 +268,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-gmres", action="store_true", help="skip the two-sphere GMRES workload")
     ap.add_argument("--no-h256", action="store_true", help="skip the h=1/256 two-sphere GMRES workload")
+    ap.add_argument("--cpu-h256", action="store_true",
+                    help="also time two CPU treecode matvecs and the RHS at h=1/256 (minutes of CPU time)")
     ap.add_argument("--gmres-only", type=int, default=0, metavar="N",
                     help="run only the two-sphere GMRES workload at h=1/N (128 or 256) and print its JSON")
     ap.add_argument("--h", type=float, default=H)
@@ -246,7 +284,8 @@ def main():
         if args.gmres_only == 128:
             print(json.dumps(gmres_workload()), flush=True)
         else:
-            print(json.dumps(gmres_workload(h=1.0 / args.gmres_only, P=10, N0=8000)), flush=True)
+            print(json.dumps(gmres_workload(h=1.0 / args.gmres_only, P=10, N0=8000, cpu=args.cpu_h256,
+                                            cpu_reps=2)), flush=True)
         return
     body, s, t, DEL = build_workload(h=args.h)
     ctx = hor.Context(0)
@@ -322,10 +361,14 @@ def main():
     gm = None
     if not args.no_gmres:
         log("two-sphere GMRES workload (h=1/128, treecode P=8 N0=4000)")
-        gm = {"h128": gmres_workload()}
+        gm = {"h128": gmres_workload(cpu=not args.no_cpu)}
+        for gap in (0.05, 0.025):
+            log(f"two-sphere GMRES workload (h=1/128, gap {gap})")
+            gm[f"h128_gap{gap}"] = gmres_workload(gap=gap)
         if not args.no_h256:
             log("two-sphere GMRES workload (h=1/256, treecode P=10 N0=8000)")
-            gm["h256"] = gmres_workload(h=1.0 / 256.0, P=10, N0=8000, direct_check=True)
+            gm["h256"] = gmres_workload(h=1.0 / 256.0, P=10, N0=8000, direct_check=True,
+                                        cpu=args.cpu_h256, cpu_reps=2)
     line = {
         "metric": "pair interactions/s",
         "value": value,

@@ -178,12 +178,13 @@ __device__ __forceinline__ void dl_off_full(const SrcRec& s, const Tgt& t, doubl
   double m1b = g_xh * n0_n + g_n0 * xh_n;
   double m2a = g_xh * xh_n;
   double m2b = t.b * m1b;
-  double t1x = dx * m1a - t.n0x * m1b;
-  double t1y = dy * m1a - t.n0y * m1b;
-  double t1z = dz * m1a - t.n0z * m1b;
-  u[0] += t1x * H2 + (-sig2 * t1x + dx * m2a + xh0 * m2b) * H3;
-  u[1] += t1y * H2 + (-sig2 * t1y + dy * m2a + xh1 * m2b) * H3;
-  u[2] += t1z * H2 + (-sig2 * t1z + dz * m2a + xh2 * m2b) * H3;
+  // t1 = d m1a - n0 m1b;  u += t1 (H2 - sig2 H3) + (d m2a + xh m2b) H3,
+  // collected per vector: u += d Bd - n0 Bn + xh Bx
+  double A1 = H2 - sig2 * H3;
+  double Bd = m1a * A1 + m2a * H3, Bn = m1b * A1, Bx = m2b * H3;
+  u[0] += dx * Bd - t.n0x * Bn + xh0 * Bx;
+  u[1] += dy * Bd - t.n0y * Bn + xh1 * Bx;
+  u[2] += dz * Bd - t.n0z * Bn + xh2 * Bx;
 }
 
 // ---- DL on-surface pair, full branch structure (stresslets.cpp:244-269).

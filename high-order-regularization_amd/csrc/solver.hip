@@ -228,23 +228,28 @@ static int matvec_dev(hor_solver* S, const double* u, double* Au) {
   return matvec_plans(S, S->on, S->off, u, Au);
 }
 
+// FP64 plans of a mixed-precision solver, for the true residual.
+static int ensure_fp64_plans(hor_solver* S) {
+  if (S->precision != HOR_MIXED || S->on64[0]) return HOR_OK;
+  hor_ctx* c = S->ctx;
+  for (int k = 0; k < 2; k++) {
+    const int o = 1 - k;
+    const int nk = (int)S->area[k].size(), no = (int)S->area[o].size();
+    HOR_TRY(hor_plan_create(c, HOR_DL_ON, S->method, HOR_FP64, S->DEL, nk, S->x[k].data(), S->nrml[k].data(),
+                            S->area[k].data(), nk, S->x[k].data(), nullptr, nullptr, nullptr, S->ctr[k],
+                            &S->tp[k], &S->on64[k]));
+    HOR_TRY(hor_plan_create(c, HOR_DL_OFF, S->method, HOR_FP64, S->DEL, nk, S->x[k].data(), S->nrml[k].data(),
+                            S->area[k].data(), no, S->x[o].data(), S->b[o].data(), S->x0[o].data(),
+                            S->n0[o].data(), S->ctr[k], &S->tp[k], &S->off64[k]));
+  }
+  return HOR_OK;
+}
+
 // The FP64 operator: the solver's own plans, or for a mixed-precision solver
-// a second set of FP64 plans built on first use.
+// its second set of FP64 plans.
 static int matvec64_dev(hor_solver* S, const double* u, double* Au) {
   if (S->precision != HOR_MIXED) return matvec_plans(S, S->on, S->off, u, Au);
-  if (!S->on64[0]) {
-    hor_ctx* c = S->ctx;
-    for (int k = 0; k < 2; k++) {
-      const int o = 1 - k;
-      const int nk = (int)S->area[k].size(), no = (int)S->area[o].size();
-      HOR_TRY(hor_plan_create(c, HOR_DL_ON, S->method, HOR_FP64, S->DEL, nk, S->x[k].data(), S->nrml[k].data(),
-                              S->area[k].data(), nk, S->x[k].data(), nullptr, nullptr, nullptr, S->ctr[k],
-                              &S->tp[k], &S->on64[k]));
-      HOR_TRY(hor_plan_create(c, HOR_DL_OFF, S->method, HOR_FP64, S->DEL, nk, S->x[k].data(),
-                              S->nrml[k].data(), S->area[k].data(), no, S->x[o].data(), S->b[o].data(),
-                              S->x0[o].data(), S->n0[o].data(), S->ctr[k], &S->tp[k], &S->off64[k]));
-    }
-  }
+  HOR_TRY(ensure_fp64_plans(S));
   return matvec_plans(S, S->on64, S->off64, u, Au);
 }
 
@@ -304,6 +309,7 @@ extern "C" int hor_solver_create(hor_ctx* ctx, double h, double DEL, int method,
     HOR_TRY(S->nrst[k].upload(me->nrst, me->n, st));
     HOR_TRY(S->g0[k].alloc(3 * (size_t)me->n));
   }
+  HOR_TRY(ensure_fp64_plans(S.get()));
   HOR_TRY(S->vin.alloc(S->n));
   HOR_TRY(S->vout.alloc(S->n));
   HOR_TRY(S->part.alloc(kRedBlocks));
