@@ -320,6 +320,15 @@ def main():
     pairs_per_step = 2.0 * ns * nt
     value = pairs_per_step / (wall / args.steps)
 
+    # the same step through the host-buffer entry points (densities in, velocities
+    # out over PCIe): reported beside the value, never as it
+    p_sl(s["f"], t["f0"])
+    t0 = time.perf_counter()
+    for _ in range(2):
+        p_sl(s["f"], t["f0"])
+        p_dl(s["g"], t["g0"])
+    host_ms = 1e3 * (time.perf_counter() - t0) / 2
+
     # mixed-precision variant (FP32 far pairs, FP64 accumulation): time and error
     m_sl = hor.Plan(ctx, hor.SL_OFF, s, t, DEL, precision=hor.MIXED)
     m_dl = hor.Plan(ctx, hor.DL_OFF, s, t, DEL, precision=hor.MIXED)
@@ -387,6 +396,8 @@ def main():
                    "sources": ns, "targets": nt, "delta": DEL, "pairs_per_step": pairs_per_step},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "host_buffers": {"ms_per_step": host_ms, "value": pairs_per_step / (host_ms * 1e-3),
+                         "note": "hor_plan_eval with host arrays: density upload, sums, velocity download"},
         "mixed": {"ms_per_step": mixed_ms, "value": pairs_per_step / (mixed_ms * 1e-3),
                   "rel_err_vs_fp64_sl": err_sl, "rel_err_vs_fp64_dl": err_dl},
         "accuracy": {"sl_max_err_inside_vs_exact_U": err_inside},
