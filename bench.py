@@ -184,6 +184,11 @@ def gmres_workload(h=1.0 / 128.0, gap=0.1, P=8, N0=4000, theta=0.6, tol=1e-8, se
         for _ in range(3):
             S.matvec(b_rigid)
         out.setdefault("matvec_ms", {})[name] = 1e3 * (time.perf_counter() - t0) / 3
+    st = S64.stats()
+    mv_s = out["matvec_ms"]["fp64"] * 1e-3
+    out["matvec_work"] = dict(st, direct_equivalent_pairs_per_s=st["direct_pairs"] / mv_s,
+                              evaluated_interactions_per_s=(st["near_pairs"] + st["proxy_pairs"] +
+                                                            st["moment_terms"]) / mv_s)
     if cpu:
         try:
             cores = len(os.sched_getaffinity(0))

@@ -8,8 +8,9 @@
 // node are tabulated.
 //
 // Evaluation (per density, all on the device):
-//   1. k_moments:  proxy weights W[c][kk] = sum_s a1_i a2_j a3_k D A q_c over
-//                  node members, split into member chunks, summed in order;
+//   1. k_moments:  proxy weights W[kk][c] = sum_s (a1_i a2_j a3_k D)(s) (A q_c)(s)
+//                  over the node's members, a GEMM on the FP64 matrix cores
+//                  (proxies x sources times sources x components);
 //   2. k_leaf:     regularized direct sums of a leaf's sources for its near
 //                  targets (same pair code and LDS tiling as the direct kernel);
 //   3. k_far:      plain Stokeslet / stresslet from the (P+1)^3 proxies with
@@ -25,7 +26,8 @@
 
 namespace hor {
 
-constexpr int kMomChunk = 1024;  // sources per moment work item
+
+constexpr int kMomChunk = 4096;  // members per moment work item
 
 struct TreePlan {
   Tree T;
@@ -43,9 +45,11 @@ struct TreePlan {
   DBuf<int> far_item_node, far_item_off, far_item_cnt, far_tgt, far_slot;
   // moments
   int nfar_nodes = 0, n_mom_items = 0;
-  DBuf<int> mom_item_node, mom_item_s0, mom_item_s1, mom_node_first, mom_node_nitems, fac_off;
-  DBuf<double> fac;          // per far node: A1 (np1*n), A2, A3, D (n)
-  DBuf<double> mom_part, mom;    // mom: per far node, proxy-major [kk][ncomp]
+  DBuf<int> far_node, fac_off, mom_item_far, mom_item_m0, mom_item_m1, mom_node_first, mom_node_nitems;
+  DBuf<double> mom_part;     // per member-chunk work item: [kk][ncomp]
+  DBuf<double> fac;          // per far node, member-major: a1 (np1), a2, a3, D per member
+  DBuf<double> mq;           // per tree slot: A q_c, 16 values (zero-padded)
+  DBuf<double> mom;          // per far node, proxy-major [kk][ncomp]
   DBuf<float> mom32;             // mixed precision: [kk][ncomp32] FP32 copy (padded)
   DBuf<float> cheb32;            // per node: Chebyshev points minus the node centre, FP32
   DBuf<double> node_ctr;         // per node: centre (midpoint of the Chebyshev interval)
@@ -58,84 +62,114 @@ struct TreePlan {
 // Device kernels
 
 struct MomArgs {
-  int np1, pflat, ncomp, nsp;
-  const int *item_node, *item_s0, *item_s1;  // per item: node, member range [s0, s1) (tree slots)
-  const int *node_lo, *fac_off, *far_idx;
-  const double* fac;
-  const double *sn, *sa, *sd;                // tree-order sources (stride nsp)
-  double* part;                              // n_items * ncomp * pflat
+  int np1, pflat, ncomp;
+  const int *item_far, *item_m0, *item_m1;  // per work item: far node index, member range [m0, m1)
+  const int *far_node, *node_lo, *fac_off;
+  const double* fac;   // member-major barycentric factors (a1, a2, a3, D) per far node
+  const double* mq;    // per tree slot: A q_c (16 values)
+  double* part;        // per work item: [kk][ncomp] partial weights
 };
 
-// Panel_Moments_SL / Panel_Moments_DL (KITC.cpp:351-668), over one member chunk.
-template <bool DL>
-__global__ __launch_bounds__(kBlock) void k_moments(MomArgs A) {
-  constexpr int NC = DL ? 12 : 6;
-  const int item = blockIdx.x;
-  const int kk = blockIdx.y * kBlock + threadIdx.x;
-  const int node = A.item_node[item];
-  const int s0 = A.item_s0[item], s1 = A.item_s1[item];
-  const int lo = A.node_lo[node];
-  const int np1 = A.np1;
-  const int nmem = A.fac_off[A.far_idx[node] + 1] - A.fac_off[A.far_idx[node]];
-  const int nm = nmem / (3 * np1 + 1);
-  const double* F = A.fac + A.fac_off[A.far_idx[node]];
-  const double* A1 = F;
-  const double* A2 = F + (size_t)np1 * nm;
-  const double* A3 = F + (size_t)2 * np1 * nm;
-  const double* D = F + (size_t)3 * np1 * nm;
-  bool act = kk < A.pflat;
-  int i = 0, j = 0, k = 0;
-  if (act) {
-    i = kk / (np1 * np1);
-    j = (kk / np1) % np1;
-    k = kk % np1;
-  }
-  double sum[NC];
-#pragma unroll
-  for (int c = 0; c < NC; c++) sum[c] = 0.0;
-  for (int sl = s0; sl < s1; sl++) {
-    const int m = sl - lo;  // member index within the node
-    const int p = sl;       // tree slot
-    double w = A1[(size_t)i * nm + m] * A2[(size_t)j * nm + m] * A3[(size_t)k * nm + m] * D[m];
-    w *= A.sa[p];
-    double nx = A.sn[p], ny = A.sn[A.nsp + p], nz = A.sn[2 * A.nsp + p];
-    double l0 = A.sd[p], l1 = A.sd[A.nsp + p], l2 = A.sd[2 * A.nsp + p];
-    if (!DL) {
-      sum[0] += w * l0; sum[1] += w * l1; sum[2] += w * l2;
-      sum[3] += w * nx; sum[4] += w * ny; sum[5] += w * nz;
-    } else {
-      double wl[3] = {w * l0, w * l1, w * l2};
-#pragma unroll
-      for (int c = 0; c < 3; c++) {
-        sum[3 * c] += wl[c] * nx;
-        sum[3 * c + 1] += wl[c] * ny;
-        sum[3 * c + 2] += wl[c] * nz;
-      }
-      sum[9] += w * nx; sum[10] += w * ny; sum[11] += w * nz;
+// A q_c of every source (tree order): SL q = (f, n), DL q = (g_l n_m, n)
+// (Panel_Moments_SL / Panel_Moments_DL, KITC.cpp:476-487, 632-651).
+__global__ void k_moment_rhs(int ns, int nsp, int dl, const double* sn, const double* sa, const double* sd,
+                             double* mq) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= ns) return;
+  const double a = sa[p];
+  const double n[3] = {sn[p], sn[nsp + p], sn[2 * nsp + p]};
+  const double d[3] = {sd[p], sd[nsp + p], sd[2 * nsp + p]};
+  double* o = mq + (size_t)p * 16;
+  if (dl) {
+    for (int l = 0; l < 3; l++)
+      for (int m = 0; m < 3; m++) o[3 * l + m] = a * d[l] * n[m];
+    for (int m = 0; m < 3; m++) o[9 + m] = a * n[m];
+    for (int c = 12; c < 16; c++) o[c] = 0.0;
+  } else {
+    for (int m = 0; m < 3; m++) {
+      o[m] = a * d[m];
+      o[3 + m] = a * n[m];
     }
-  }
-  if (act) {
-    double* P = A.part + (size_t)item * NC * A.pflat;
-#pragma unroll
-    for (int c = 0; c < NC; c++) P[(size_t)c * A.pflat + kk] = sum[c];
+    for (int c = 6; c < 16; c++) o[c] = 0.0;
   }
 }
 
-// Sums the member-chunk partials of each far node in chunk order and stores
-// the moments proxy-major (mom[f][kk][c]), so that the far-field kernel reads
-// one proxy's components with one wide scalar load; the mixed-precision copy
-// mom32[f][kk][c] is padded to ncomp32 floats per proxy.
-__global__ void k_moments_reduce(int nfar, int ncomp, int pflat, const int* node_first, const int* node_nitems,
+typedef double f64x4_t __attribute__((ext_vector_type(4)));
+
+// Proxy weights (Panel_Moments_*, KITC.cpp:351-668) as a GEMM on the FP64
+// matrix cores: W[kk][c] = sum_s B[kk][s] Q[s][c] with B[kk][s] =
+// a1_i a2_j a3_k D (kk = (i, j, k)) formed in registers and Q = A q.  A work
+// item is a far node's member chunk; each wave owns 16 proxies and runs
+// through the chunk four members per v_mfma_f64_16x16x4f64 (lane l holds
+// B[l%16][l/16] and Q[l/16][l%16]; accumulator r of lane l is
+// W[l/16 + 4 r][l%16]), alternating two accumulators so that consecutive
+// products do not wait on each other.
+__global__ __launch_bounds__(kBlock) void k_moments(MomArgs A) {
+  const int item = blockIdx.x;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int kk0 = (blockIdx.y * 4 + wave) * 16;
+  if (kk0 >= A.pflat) return;  // wave-uniform
+  const int f = A.item_far[item];
+  const int node = A.far_node[f];
+  const int lo = A.node_lo[node];
+  const int c0 = A.item_m0[item], c1 = A.item_m1[item];  // member range within the node
+  const int np1 = A.np1, stride = 3 * np1 + 1;
+  // rows past pflat compute a clamped proxy's weights and are not stored
+  const int row = min(kk0 + (lane & 15), A.pflat - 1);
+  const int i = row / (np1 * np1), j = (row / np1) % np1, k = row % np1;
+  const int ms = lane >> 4, col = lane & 15;
+  const double* fm = A.fac + A.fac_off[f] + (size_t)(c0 + ms) * stride;
+  const double* qm = A.mq + (size_t)(lo + c0 + ms) * 16 + col;
+  const size_t fstep = (size_t)4 * stride;
+  f64x4_t acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
+  const int n = c1 - c0;
+  const int nmain = n & ~3;  // members in whole groups of four: no bounds tests
+  int m0 = 0;
+  for (; m0 + 8 <= nmain; m0 += 8) {
+    const double* g = fm + fstep;
+    const double b0 = fm[i] * fm[np1 + j] * fm[2 * np1 + k] * fm[3 * np1];
+    const double b1 = g[i] * g[np1 + j] * g[2 * np1 + k] * g[3 * np1];
+    const double q0 = qm[0], q1 = qm[64];
+    acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, q0, acc0, 0, 0, 0);
+    acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, q1, acc1, 0, 0, 0);
+    fm += 2 * fstep;
+    qm += 128;
+  }
+  if (m0 < nmain) {
+    const double b0 = fm[i] * fm[np1 + j] * fm[2 * np1 + k] * fm[3 * np1];
+    acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, qm[0], acc0, 0, 0, 0);
+    fm += fstep;
+    qm += 64;
+  }
+  if (nmain < n) {  // the last 1-3 members; the other lanes contribute zero
+    const bool in = nmain + ms < n;
+    const double bv = in ? fm[i] * fm[np1 + j] * fm[2 * np1 + k] * fm[3 * np1] : 0.0;
+    const double qv = in ? *qm : 0.0;
+    acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(bv, qv, acc1, 0, 0, 0);
+  }
+  double* P = A.part + (size_t)item * A.pflat * A.ncomp;
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const int kk = kk0 + (lane >> 4) + 4 * r;
+    if (kk < A.pflat && col < A.ncomp) P[(size_t)kk * A.ncomp + col] = acc0[r] + acc1[r];
+  }
+}
+
+// Sums the member-chunk partials of each far node in chunk order into the
+// proxy-major moments (and their FP32 copy, padded to ncomp32 per proxy).
+__global__ void k_moments_reduce(int ncomp, int pflat, const int* node_first, const int* node_nitems,
                                  const double* part, double* mom, float* mom32, int ncomp32) {
-  int f = blockIdx.x;
-  int total = ncomp * pflat;
+  const int f = blockIdx.x;
+  const int total = ncomp * pflat;
+  const int it0 = node_first[f], nit = node_nitems[f];
   for (int e = threadIdx.x; e < total; e += blockDim.x) {
     double s = 0.0;
-    int it0 = node_first[f], nit = node_nitems[f];
     for (int q = 0; q < nit; q++) s += part[(size_t)(it0 + q) * total + e];
-    const int c = e / pflat, kk = e - c * pflat;
-    mom[(size_t)f * total + (size_t)kk * ncomp + c] = s;
-    if (mom32) mom32[((size_t)f * pflat + kk) * ncomp32 + c] = (float)s;
+    mom[(size_t)f * total + e] = s;
+    if (mom32) {
+      const int kk = e / ncomp, c = e - kk * ncomp;
+      mom32[((size_t)f * pflat + kk) * ncomp32 + c] = (float)s;
+    }
   }
 }
 
@@ -477,35 +511,27 @@ int tree_plan_create(hor_plan* P, const double* sx, const double* sn, const doub
     fac_off[f + 1] = fac_off[f] + (hi[q] - lo[q] + 1) * (3 * TP.np1 + 1);
   }
   std::vector<double> fac(std::max<size_t>(fac_off[TP.nfar_nodes], 1), 0.0);
-  std::vector<int> mi_node, mi_s0, mi_s1, m_first(TP.nfar_nodes), m_nit(TP.nfar_nodes);
   double near_pairs = 0, moment_terms = 0;
+  std::vector<int> mi_far, mi_m0, mi_m1, m_first(TP.nfar_nodes), m_nit(TP.nfar_nodes);
   for (int f = 0; f < TP.nfar_nodes; f++) {
-    int q = far_nodes[f], nm = hi[q] - lo[q] + 1, np1 = TP.np1;
+    const int q = far_nodes[f], nm = hi[q] - lo[q] + 1, np1 = TP.np1, stride = 3 * np1 + 1;
+    m_first[f] = (int)mi_far.size();
+    for (int a = 0; a < nm; a += kMomChunk) {
+      mi_far.push_back(f);
+      mi_m0.push_back(a);
+      mi_m1.push_back(std::min(nm, a + kMomChunk));
+    }
+    m_nit[f] = (int)mi_far.size() - m_first[f];
     double* F = &fac[fac_off[f]];
     const double* cq = &T.cheb[3 * (size_t)np1 * q];
-    std::vector<double> a1(np1), a2(np1), a3(np1);
     for (int m = 0; m < nm; m++) {
-      int o = T.perm[lo[q] + m];
-      double Dv;
-      bary_factors(tp.P, cq, cq + np1, cq + 2 * np1, sx[3 * o], sx[3 * o + 1], sx[3 * o + 2], a1.data(),
-                   a2.data(), a3.data(), &Dv);
-      for (int j = 0; j < np1; j++) {
-        F[(size_t)j * nm + m] = a1[j];
-        F[(size_t)(np1 + j) * nm + m] = a2[j];
-        F[(size_t)(2 * np1 + j) * nm + m] = a3[j];
-      }
-      F[(size_t)3 * np1 * nm + m] = Dv;
+      const int o = T.perm[lo[q] + m];
+      double* fm = F + (size_t)m * stride;
+      bary_factors(tp.P, cq, cq + np1, cq + 2 * np1, sx[3 * o], sx[3 * o + 1], sx[3 * o + 2], fm, fm + np1,
+                   fm + 2 * np1, fm + 3 * np1);
     }
-    m_first[f] = (int)mi_node.size();
-    for (int a = lo[q]; a <= hi[q]; a += kMomChunk) {
-      mi_node.push_back(q);
-      mi_s0.push_back(a);
-      mi_s1.push_back(std::min(hi[q] + 1, a + kMomChunk));
-    }
-    m_nit[f] = (int)mi_node.size() - m_first[f];
     moment_terms += (double)nm * TP.pflat;
   }
-  TP.n_mom_items = (int)mi_node.size();
   for (int q = 0; q < nn; q++) near_pairs += (double)ntg[q].size() * (hi[q] - lo[q] + 1);
   P->near_pairs = near_pairs;
   P->proxy_pairs = (double)f_tg.size() * TP.pflat;
@@ -544,14 +570,17 @@ int tree_plan_create(hor_plan* P, const double* sx, const double* sn, const doub
   HOR_TRY(TP.far_item_cnt.upload(fi_cnt, s));
   HOR_TRY(TP.far_tgt.upload(f_tg, s));
   HOR_TRY(TP.far_slot.upload(f_sl, s));
-  HOR_TRY(TP.mom_item_node.upload(mi_node, s));
-  HOR_TRY(TP.mom_item_s0.upload(mi_s0, s));
-  HOR_TRY(TP.mom_item_s1.upload(mi_s1, s));
+  HOR_TRY(TP.far_node.upload(far_nodes.empty() ? std::vector<int>{0} : far_nodes, s));
+  TP.n_mom_items = (int)mi_far.size();
+  HOR_TRY(TP.mom_item_far.upload(mi_far, s));
+  HOR_TRY(TP.mom_item_m0.upload(mi_m0, s));
+  HOR_TRY(TP.mom_item_m1.upload(mi_m1, s));
   HOR_TRY(TP.mom_node_first.upload(m_first, s));
   HOR_TRY(TP.mom_node_nitems.upload(m_nit, s));
+  HOR_TRY(TP.mom_part.alloc((size_t)std::max(TP.n_mom_items, 1) * TP.ncomp * TP.pflat));
   HOR_TRY(TP.fac_off.upload(fac_off, s));
   HOR_TRY(TP.fac.upload(fac, s));
-  HOR_TRY(TP.mom_part.alloc((size_t)std::max(TP.n_mom_items, 1) * TP.ncomp * TP.pflat));
+  HOR_TRY(TP.mq.alloc((size_t)P->nsp * 16));
   HOR_TRY(TP.mom.alloc((size_t)std::max(TP.nfar_nodes, 1) * TP.ncomp * TP.pflat));
   if (P->precision == HOR_MIXED)
     HOR_TRY(TP.mom32.alloc((size_t)std::max(TP.nfar_nodes, 1) * (dl ? kNc32Dl : kNc32Sl) * TP.pflat));
@@ -589,17 +618,18 @@ int tree_plan_eval(hor_plan* P, const double* dens_dev, const double* d0_dev, do
     HOR_HIP_TRY(hipMemcpyAsync(P->tsub2.p, P->tsub.p, sizeof(double) * 3 * P->ntp, hipMemcpyDeviceToDevice, s));
   HOR_HIP_TRY(hipEventRecord(c->ev[2], s));
   // 1. moments
-  if (TP.n_mom_items > 0) {
+  if (TP.nfar_nodes > 0) {
+    hipLaunchKernelGGL(k_moment_rhs, dim3((P->ns + kBlock - 1) / kBlock), dim3(kBlock), 0, s, P->ns, P->nsp,
+                       dl ? 1 : 0, P->sn.p, P->sa.p, P->sd.p, TP.mq.p);
     MomArgs M;
-    M.np1 = TP.np1; M.pflat = TP.pflat; M.ncomp = TP.ncomp; M.nsp = P->nsp;
-    M.item_node = TP.mom_item_node.p; M.item_s0 = TP.mom_item_s0.p; M.item_s1 = TP.mom_item_s1.p;
-    M.node_lo = TP.node_lo.p; M.fac_off = TP.fac_off.p; M.far_idx = TP.far_idx.p; M.fac = TP.fac.p;
-    M.sn = P->sn.p; M.sa = P->sa.p; M.sd = P->sd.p; M.part = TP.mom_part.p;
-    dim3 grid(TP.n_mom_items, (TP.pflat + kBlock - 1) / kBlock);
-    if (dl) hipLaunchKernelGGL((k_moments<true>), grid, dim3(kBlock), 0, s, M);
-    else hipLaunchKernelGGL((k_moments<false>), grid, dim3(kBlock), 0, s, M);
-    hipLaunchKernelGGL(k_moments_reduce, dim3(TP.nfar_nodes), dim3(kBlock), 0, s, TP.nfar_nodes, TP.ncomp,
-                       TP.pflat, TP.mom_node_first.p, TP.mom_node_nitems.p, TP.mom_part.p, TP.mom.p,
+    M.np1 = TP.np1; M.pflat = TP.pflat; M.ncomp = TP.ncomp;
+    M.item_far = TP.mom_item_far.p; M.item_m0 = TP.mom_item_m0.p; M.item_m1 = TP.mom_item_m1.p;
+    M.far_node = TP.far_node.p; M.node_lo = TP.node_lo.p; M.fac_off = TP.fac_off.p;
+    M.fac = TP.fac.p; M.mq = TP.mq.p; M.part = TP.mom_part.p;
+    const int tiles = (TP.pflat + 15) / 16;
+    hipLaunchKernelGGL(k_moments, dim3(TP.n_mom_items, (tiles + 3) / 4), dim3(kBlock), 0, s, M);
+    hipLaunchKernelGGL(k_moments_reduce, dim3(TP.nfar_nodes), dim3(kBlock), 0, s, TP.ncomp, TP.pflat,
+                       TP.mom_node_first.p, TP.mom_node_nitems.p, TP.mom_part.p, TP.mom.p,
                        mixed ? TP.mom32.p : nullptr, dl ? kNc32Dl : kNc32Sl);
     HOR_HIP_TRY(hipGetLastError());
   }

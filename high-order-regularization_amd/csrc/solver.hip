@@ -327,6 +327,24 @@ extern "C" int hor_solver_destroy(hor_solver* S) {
   return HOR_OK;
 }
 
+// Work of one matvec, summed over its four DL plans (hor_plan_stats).
+extern "C" int hor_solver_stats(hor_solver* S, double* direct_pairs, double* near_pairs, double* proxy_pairs,
+                                double* moment_terms) {
+  if (!S) return set_error(HOR_E_ARG, "hor_solver_stats: null");
+  double acc[4] = {0, 0, 0, 0};
+  hor_plan* plans[4] = {S->on[0], S->on[1], S->off[0], S->off[1]};
+  for (hor_plan* P : plans) {
+    double v[4];
+    HOR_TRY(hor_plan_stats(P, &v[0], &v[1], &v[2], &v[3]));
+    for (int k = 0; k < 4; k++) acc[k] += v[k];
+  }
+  if (direct_pairs) *direct_pairs = acc[0];
+  if (near_pairs) *near_pairs = acc[1];
+  if (proxy_pairs) *proxy_pairs = acc[2];
+  if (moment_terms) *moment_terms = acc[3];
+  return HOR_OK;
+}
+
 extern "C" int hor_solver_size(hor_solver* S, int* n) {
   if (!S || !n) return set_error(HOR_E_ARG, "null");
   *n = S->n;

@@ -59,6 +59,13 @@ class Solver:
         except Exception:
             pass
 
+    def stats(self):
+        """Work of one matvec (hor_solver_stats)."""
+        v = [C.c_double() for _ in range(4)]
+        check(_lib.lib().hor_solver_stats(self.h, *[C.byref(x) for x in v]), "hor_solver_stats")
+        return dict(direct_pairs=v[0].value, near_pairs=v[1].value, proxy_pairs=v[2].value,
+                    moment_terms=v[3].value)
+
     def matvec(self, u):
         u = f64(u, self.n)
         out = np.zeros(self.n)

@@ -202,6 +202,11 @@ int hor_solver_create(hor_ctx* ctx, double h, double DEL, int method, int precis
 int hor_solver_destroy(hor_solver* solver);
 /* n = 3*(n1+n2) */
 int hor_solver_size(hor_solver* solver, int* n);
+/* Work of one matvec summed over its four DL operators (see hor_plan_stats):
+ * direct-sum-equivalent pairs, near (leaf or direct) pairs, proxy
+ * interactions, moment terms. */
+int hor_solver_stats(hor_solver* solver, double* direct_pairs, double* near_pairs,
+                     double* proxy_pairs, double* moment_terms);
 
 /* solve_IE_1step (main.cpp:541-677): Au = u - 2/3 [DL11+DL21; DL12+DL22](u),
  * with g0 interpolated on the 6x6 Monge stencil (main.cpp:916-1046).  Host
