@@ -15,12 +15,14 @@ struct DirectArgs {
   int nsub, subs_per_chunk;        // 64-source subtiles, subtiles per grid.y chunk
   double DEL, far_d2;              // delta, (8 delta (1+1e-10))^2
   double near_r2;                  // (8 delta)^2 (1+1e-12): pairs beyond are plain for sure
+  double near64_r2;                // (4 delta)^2: mixed precision keeps pairs within it in FP64
   const double* sx;                // 3*nsp positions
   const double* sn;                // 3*nsp normals
   const double* sa;                // nsp weights
   const double* sd;                // 3*nsp density (f or g)
   const struct SrcRec* rec;        // nsp source records (sorted), this evaluation's density
-  const struct SrcRec32* rec32;    // FP32 copies (mixed precision) or null
+  const struct SrcRec32* rec32;    // FP32 copies (mixed precision) or null; direct plans store
+                                   // y - (centre of the source's 64-subtile box)
   const double* sub_box;           // 6 per 64-source subtile (lo xyz, hi xyz)
   const double* tx;                // 3*ntp positions
   const double* tcoef;             // 12*ntp regularization polynomial coefficients
@@ -40,7 +42,7 @@ int launch_finalize(const double* partial, int nchunk, int nt, int ntp, double s
                     const double* base, double coef, int accumulate, hipStream_t s);
 int launch_gather3(const double* dens, const int* perm, int n, int np, const double* sx, const double* sn,
                    const double* sa, double* sd, struct SrcRec* rec, struct SrcRec32* rec32,
-                   int scale_density, hipStream_t s);
+                   const double* sub_box, int scale_density, hipStream_t s);
 int launch_target_sub(int mode, int nt, int ntp, const int* tperm, const int* tnear,
                       const double* d0, const double* n0_orig, const double* dens,
                       const double* sn_orig, double* out, hipStream_t s);

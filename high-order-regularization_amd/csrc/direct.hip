@@ -29,12 +29,11 @@ __global__ __launch_bounds__(kBlock) void k_direct(DirectArgs A) {
   load_target<KIND>(A, i, t);
   const double* wb = A.wave_box + 8 * wave;
   const bool all_far = wb[6] != 0.0;
-  float pxf = (float)t.px, pyf = (float)t.py, pzf = (float)t.pz;
   double u[3] = {0.0, 0.0, 0.0};
   // this chunk's 64-source subtiles
   const int sub0 = blockIdx.y * A.subs_per_chunk;
   const int sub1 = min(A.nsub, sub0 + A.subs_per_chunk);
-  range_sum<KIND, MIXED>(A, sub0 * kWave, min(A.ns, sub1 * kWave), sub0, wb, all_far, t, pxf, pyf, pzf, u);
+  range_sum<KIND, MIXED, true>(A, sub0 * kWave, min(A.ns, sub1 * kWave), sub0, wb, all_far, t, u);
   double* P = A.partial + (size_t)blockIdx.y * 3 * A.ntp;
   P[i] = u[0];
   P[A.ntp + i] = u[1];
@@ -68,7 +67,8 @@ __global__ void k_direct_finalize(const double* partial, int nchunk, int nt, int
 // density) for the sum kernels, plus the density as sorted SoA for the
 // treecode moments: rec[j] <- geometry[j], dens[3*perm[j] + k].
 __global__ void k_gather3(const double* dens, const int* perm, int n, int np, const double* sx, const double* sn,
-                          const double* sa, double* sd, SrcRec* rec, SrcRec32* rec32, int scale_density) {
+                          const double* sa, double* sd, SrcRec* rec, SrcRec32* rec32, const double* sub_box,
+                          int scale_density) {
   int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
   int o = perm[j];
@@ -86,8 +86,15 @@ __global__ void k_gather3(const double* dens, const int* perm, int n, int np, co
   }
   rec[j] = r;
   if (rec32) {
+    // positions relative to the centre of the source's 64-subtile box (the
+    // direct sums' FP32 plain loop), absolute otherwise
+    double c[3] = {0.0, 0.0, 0.0};
+    if (sub_box) {
+      const double* b = sub_box + 6 * (j / kWave);
+      for (int k = 0; k < 3; k++) c[k] = 0.5 * (b[k] + b[3 + k]);
+    }
     SrcRec32 q;
-    q.x = (float)r.x; q.y = (float)r.y; q.z = (float)r.z; q.a = (float)r.a;
+    q.x = (float)(r.x - c[0]); q.y = (float)(r.y - c[1]); q.z = (float)(r.z - c[2]); q.a = (float)r.a;
     q.nx = (float)r.nx; q.ny = (float)r.ny; q.nz = (float)r.nz; q.pad0 = 0.0f;
     q.d0 = (float)r.d0; q.d1 = (float)r.d1; q.d2 = (float)r.d2; q.pad1 = 0.0f;
     rec32[j] = q;
@@ -175,11 +182,11 @@ int launch_finalize(const double* partial, int nchunk, int nt, int ntp, double s
 }
 
 int launch_gather3(const double* dens, const int* perm, int n, int np, const double* sx, const double* sn,
-                   const double* sa, double* sd, SrcRec* rec, SrcRec32* rec32, int scale_density,
-                   hipStream_t s) {
+                   const double* sa, double* sd, SrcRec* rec, SrcRec32* rec32, const double* sub_box,
+                   int scale_density, hipStream_t s) {
   if (n == 0) return HOR_OK;
   hipLaunchKernelGGL(k_gather3, dim3((n + 255) / 256), dim3(256), 0, s, dens, perm, n, np, sx, sn, sa, sd, rec, rec32,
-                     scale_density);
+                     sub_box, scale_density);
   HOR_HIP_TRY(hipGetLastError());
   return HOR_OK;
 }

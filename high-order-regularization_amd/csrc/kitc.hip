@@ -208,10 +208,9 @@ __global__ __launch_bounds__(kBlock) void k_leaf(LeafArgs A) {
   wb[3] = wave_max(t.px); wb[4] = wave_max(t.py); wb[5] = wave_max(t.pz);
   const bool off_kind = (KIND == HOR_SL_OFF || KIND == HOR_DL_OFF);
   const bool all_far = off_kind && !__any(t.nearp ? 1 : 0);
-  float pxf = (float)t.px, pyf = (float)t.py, pzf = (float)t.pz;
   double u[3] = {0.0, 0.0, 0.0};
   const int lo = A.node_lo[node], hi = A.node_hi[node];
-  range_sum<KIND, MIXED>(A.D, lo, hi + 1, A.leaf_sub_off[node], wb, all_far, t, pxf, pyf, pzf, u);
+  range_sum<KIND, MIXED, false>(A.D, lo, hi + 1, A.leaf_sub_off[node], wb, all_far, t, u);
   if (act) {
     int s = A.slot[li];
     A.slots[s] = u[0];
@@ -606,7 +605,7 @@ int tree_plan_eval(hor_plan* P, const double* dens_dev, const double* d0_dev, do
   const bool dl = (P->kind == HOR_DL_ON || P->kind == HOR_DL_OFF);
   // density in tree order; per-target subtraction data (leaf and proxy variants)
   HOR_TRY(launch_gather3(dens_dev, P->sperm.p, P->ns, P->nsp, P->sx.p, P->sn.p, P->sa.p, P->sd.p, P->rec.p,
-                         mixed ? P->rec32.p : nullptr,
+                         mixed ? P->rec32.p : nullptr, nullptr,
                          P->kind == HOR_SL_ON || P->kind == HOR_SL_OFF, s));
   HOR_TRY(prepare_targets(P, dens_dev, d0_dev));
   if (P->kind == HOR_SL_OFF)

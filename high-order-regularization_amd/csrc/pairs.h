@@ -35,30 +35,28 @@ struct Tgt {
 
 // ---- SL pair: plain kernel with subtraction a [f - (f0.n0) n] (fast path):
 //   u += a f~ / r + dx (a f~ . dx) / r^3  =  (a f~ + dx t) / r,  t = (a f~ . dx) / r^2
-template <bool F32>
-__device__ __forceinline__ void sl_plain(const SrcRec& s, const SrcRec32& s32, double px, double py,
-                                         double pz, float pxf, float pyf, float pzf, double sub,
+__device__ __forceinline__ void sl_plain(const SrcRec& s, double px, double py, double pz, double sub,
                                          double u[3]) {
-  if (F32) {
-    float dx = pxf - s32.x, dy = pyf - s32.y, dz = pzf - s32.z;
-    float r2 = dx * dx + dy * dy + dz * dz;
-    float ri = rsqrt32(r2);
-    float subf = (float)sub;
-    float fx = s32.d0 - subf * s32.nx, fy = s32.d1 - subf * s32.ny, fz = s32.d2 - subf * s32.nz;
-    float t = (fx * dx + fy * dy + fz * dz) * (ri * ri);
-    u[0] += (double)((fx + dx * t) * ri);
-    u[1] += (double)((fy + dy * t) * ri);
-    u[2] += (double)((fz + dz * t) * ri);
-  } else {
-    double dx = px - s.x, dy = py - s.y, dz = pz - s.z;
-    double r2 = dx * dx + dy * dy + dz * dz;
-    double ri = rsqrt64(r2);
-    double fx = s.d0 - sub * s.nx, fy = s.d1 - sub * s.ny, fz = s.d2 - sub * s.nz;
-    double t = (fx * dx + fy * dy + fz * dz) * (ri * ri);
-    u[0] += (fx + dx * t) * ri;
-    u[1] += (fy + dy * t) * ri;
-    u[2] += (fz + dz * t) * ri;
-  }
+  double dx = px - s.x, dy = py - s.y, dz = pz - s.z;
+  double r2 = dx * dx + dy * dy + dz * dz;
+  double ri = rsqrt64(r2);
+  double fx = s.d0 - sub * s.nx, fy = s.d1 - sub * s.ny, fz = s.d2 - sub * s.nz;
+  double t = (fx * dx + fy * dy + fz * dz) * (ri * ri);
+  u[0] += (fx + dx * t) * ri;
+  u[1] += (fy + dy * t) * ri;
+  u[2] += (fz + dz * t) * ri;
+}
+
+// The same in FP32 for a given separation d (FP32 partial sums).
+__device__ __forceinline__ void sl_plain32(const SrcRec32& s, float dx, float dy, float dz, float sub,
+                                           float u[3]) {
+  float r2 = dx * dx + dy * dy + dz * dz;
+  float ri = rsqrt32(r2);
+  float fx = s.d0 - sub * s.nx, fy = s.d1 - sub * s.ny, fz = s.d2 - sub * s.nz;
+  float t = (fx * dx + fy * dy + fz * dz) * (ri * ri);
+  u[0] += (fx + dx * t) * ri;
+  u[1] += (fy + dy * t) * ri;
+  u[2] += (fz + dz * t) * ri;
 }
 
 // ---- SL pair, full branch structure (mixed path), FP64
@@ -103,33 +101,32 @@ __device__ __forceinline__ void sl_full(const SrcRec& s, double px, double py, d
 // target within 8 delta of the surface this is the reference's
 // subtraction form (stresslets.cpp:157-167) evaluated in the algebraically
 // identical, cancellation-free product form.
-template <bool F32>
-__device__ __forceinline__ void dl_plain(const SrcRec& s, const SrcRec32& s32, double px, double py,
-                                         double pz, float pxf, float pyf, float pzf, double g0x,
+__device__ __forceinline__ void dl_plain(const SrcRec& s, double px, double py, double pz, double g0x,
                                          double g0y, double g0z, double u[3]) {
-  if (F32) {
-    float dx = pxf - s32.x, dy = pyf - s32.y, dz = pzf - s32.z;
-    float r2 = dx * dx + dy * dy + dz * dz;
-    float ri = rsqrt32(r2);
-    float ri2 = ri * ri;
-    float gdx = (s32.d0 - (float)g0x) * dx + (s32.d1 - (float)g0y) * dy + (s32.d2 - (float)g0z) * dz;
-    float ndx = s32.nx * dx + s32.ny * dy + s32.nz * dz;
-    float k = gdx * ndx * (ri2 * ri2 * ri);
-    u[0] += (double)(dx * k);
-    u[1] += (double)(dy * k);
-    u[2] += (double)(dz * k);
-  } else {
-    double dx = px - s.x, dy = py - s.y, dz = pz - s.z;
-    double r2 = dx * dx + dy * dy + dz * dz;
-    double ri = rsqrt64(r2);
-    double ri2 = ri * ri;
-    double gdx = (s.d0 - g0x) * dx + (s.d1 - g0y) * dy + (s.d2 - g0z) * dz;
-    double ndx = s.nx * dx + s.ny * dy + s.nz * dz;
-    double k = gdx * ndx * (ri2 * ri2 * ri);
-    u[0] += dx * k;
-    u[1] += dy * k;
-    u[2] += dz * k;
-  }
+  double dx = px - s.x, dy = py - s.y, dz = pz - s.z;
+  double r2 = dx * dx + dy * dy + dz * dz;
+  double ri = rsqrt64(r2);
+  double ri2 = ri * ri;
+  double gdx = (s.d0 - g0x) * dx + (s.d1 - g0y) * dy + (s.d2 - g0z) * dz;
+  double ndx = s.nx * dx + s.ny * dy + s.nz * dz;
+  double k = gdx * ndx * (ri2 * ri2 * ri);
+  u[0] += dx * k;
+  u[1] += dy * k;
+  u[2] += dz * k;
+}
+
+// The same in FP32 for a given separation d (FP32 partial sums).
+__device__ __forceinline__ void dl_plain32(const SrcRec32& s, float dx, float dy, float dz, float g0x, float g0y,
+                                           float g0z, float u[3]) {
+  float r2 = dx * dx + dy * dy + dz * dz;
+  float ri = rsqrt32(r2);
+  float ri2 = ri * ri;
+  float gdx = (s.d0 - g0x) * dx + (s.d1 - g0y) * dy + (s.d2 - g0z) * dz;
+  float ndx = s.nx * dx + s.ny * dy + s.nz * dz;
+  float k = gdx * ndx * (ri2 * ri2 * ri);
+  u[0] += dx * k;
+  u[1] += dy * k;
+  u[2] += dz * k;
 }
 
 // ---- DL off-surface pair, full branch structure (stresslets.cpp:117-168).
@@ -217,6 +214,130 @@ __device__ __forceinline__ void dl_on_full(const SrcRec& s, const Tgt& t, double
   u[2] += dz * H3;
 }
 
+// ---------------------------------------------------------------------------
+// Mixed precision: regularized pairs with 4 delta < r <= 8 delta in FP32.
+// Every lane of the wave is beyond 4 delta of the source when these run, so
+// there is no self term, erf(rho) is within 2e-8 of 1 and the correction
+// terms P(rho^2) rho exp(-rho^2) are below 3e-2 of the kernel.
+
+// Per-target data of the FP32 pairs (converted once per target).
+struct Tgt32 {
+  float sub;                 // SL: f0.n0
+  float g0x, g0y, g0z;       // DL: g0
+  float ex, ey, ez;          // DL_OFF: p - x0, so that x^ = y - x0 = e - d
+  float n0x, n0y, n0z, b;    // DL_OFF
+  float q[kNumCoef];
+};
+
+__device__ __forceinline__ void make_tgt32(const Tgt& t, Tgt32& v) {
+  v.sub = (float)t.sub;
+  v.g0x = (float)t.g0x; v.g0y = (float)t.g0y; v.g0z = (float)t.g0z;
+  v.ex = (float)(t.px - t.x0x); v.ey = (float)(t.py - t.x0y); v.ez = (float)(t.pz - t.x0z);
+  v.n0x = (float)t.n0x; v.n0y = (float)t.n0y; v.n0z = (float)t.n0z; v.b = (float)t.b;
+#pragma unroll
+  for (int k = 0; k < kNumCoef; k++) v.q[k] = (float)t.q[k];
+}
+
+// erf(rho) for 4 < rho <= 8 from E = exp(-rho^2): 1 - E erfcx(rho), with the
+// asymptotic erfcx(rho) ~ (1 - 1/(2 rho^2) + 3/(4 rho^4)) / (sqrt(pi) rho),
+// whose relative error (below 5e-4 at rho = 4) moves erf by less than 1e-11.
+__device__ __forceinline__ float erf_tail32(float rho, float E) {
+  const float ir = __builtin_amdgcn_rcpf(rho);
+  const float ir2 = ir * ir;
+  return 1.0f - E * (0.5641895835f * ir) * (1.0f + ir2 * (-0.5f + 0.75f * ir2));
+}
+
+// exp(-x) in FP32 for 0 <= x <= 64 (v_exp_f32 of -x log2 e).
+__device__ __forceinline__ float expneg32(float x) { return __builtin_amdgcn_exp2f(-1.4426950408889634f * x); }
+
+__device__ __forceinline__ void sl_reg32(const SrcRec32& s, float dx, float dy, float dz, const Tgt32& v,
+                                         bool nearp, float inv_del, float u[3]) {
+  const float r2 = dx * dx + dy * dy + dz * dz;
+  const float ri = rsqrt32(r2);
+  const float rd = (r2 * ri) * inv_del;
+  const float ri3 = ri * ri * ri;
+  const float fx = s.d0 - v.sub * s.nx, fy = s.d1 - v.sub * s.ny, fz = s.d2 - v.sub * s.nz;
+  const float fdx = fx * dx + fy * dy + fz * dz;
+  float s1 = ri, s2 = ri3;
+  if (nearp && !(rd > 8.0f)) {
+    const float rd2 = rd * rd;
+    const float E = expneg32(rd2);
+    const float ef = erf_tail32(rd, E);
+    const float rex = rd * E;
+    const float p1 = v.q[0] + rd2 * (v.q[1] + rd2 * v.q[2]);
+    const float p2 = v.q[3] + rd2 * (v.q[4] + rd2 * (v.q[5] + rd2 * v.q[6]));
+    s1 = (ef + p1 * rex) * ri;
+    s2 = (ef + p2 * rex) * ri3;
+  }
+  const float H2 = s2 * fdx;
+  u[0] += fx * s1 + dx * H2;
+  u[1] += fy * s1 + dy * H2;
+  u[2] += fz * s1 + dz * H2;
+}
+
+__device__ __forceinline__ void dl_off_reg32(const SrcRec32& s, float dx, float dy, float dz, const Tgt32& v,
+                                             bool nearp, float inv_del, float u[3]) {
+  const float r2 = dx * dx + dy * dy + dz * dz;
+  const float ri = rsqrt32(r2);
+  const float rd = (r2 * ri) * inv_del;
+  const float ri2 = ri * ri;
+  const float gx = s.d0 - v.g0x, gy = s.d1 - v.g0y, gz = s.d2 - v.g0z;
+  if (!nearp || rd > 8.0f) {
+    const float gdx = gx * dx + gy * dy + gz * dz;
+    const float ndx = s.nx * dx + s.ny * dy + s.nz * dz;
+    const float k = gdx * ndx * (ri2 * ri2 * ri);
+    u[0] += dx * k;
+    u[1] += dy * k;
+    u[2] += dz * k;
+    return;
+  }
+  const float ri3 = ri2 * ri;
+  const float rd2 = rd * rd;
+  const float E = expneg32(rd2);
+  const float ef = erf_tail32(rd, E);
+  const float rex = rd * E;
+  const float p2 = v.q[3] + rd2 * (v.q[4] + rd2 * (v.q[5] + rd2 * v.q[6]));
+  const float p3 = v.q[7] + rd2 * (v.q[8] + rd2 * (v.q[9] + rd2 * (v.q[10] + rd2 * v.q[11])));
+  const float H2 = (ef + p2 * rex) * ri3;
+  const float H3 = (ef + p3 * rex) * (ri3 * ri2);
+  const float xh0 = v.ex - dx, xh1 = v.ey - dy, xh2 = v.ez - dz;
+  const float sig2 = (xh0 * xh0 + xh1 * xh1 + xh2 * xh2) - 2.0f * v.b * (xh0 * v.n0x + xh1 * v.n0y + xh2 * v.n0z);
+  const float g_n0 = gx * v.n0x + gy * v.n0y + gz * v.n0z;
+  const float g_xh = gx * xh0 + gy * xh1 + gz * xh2;
+  const float n0_n = v.n0x * s.nx + v.n0y * s.ny + v.n0z * s.nz;
+  const float xh_n = xh0 * s.nx + xh1 * s.ny + xh2 * s.nz;
+  const float m1a = g_n0 * n0_n;
+  const float m1b = g_xh * n0_n + g_n0 * xh_n;
+  const float m2a = g_xh * xh_n;
+  const float m2b = v.b * m1b;
+  const float A1 = H2 - sig2 * H3;
+  const float Bd = m1a * A1 + m2a * H3, Bn = m1b * A1, Bx = m2b * H3;
+  u[0] += dx * Bd - v.n0x * Bn + xh0 * Bx;
+  u[1] += dy * Bd - v.n0y * Bn + xh1 * Bx;
+  u[2] += dz * Bd - v.n0z * Bn + xh2 * Bx;
+}
+
+__device__ __forceinline__ void dl_on_reg32(const SrcRec32& s, float dx, float dy, float dz, const Tgt32& v,
+                                            float inv_del, float u[3]) {
+  const float r2 = dx * dx + dy * dy + dz * dz;
+  const float ri = rsqrt32(r2);
+  const float rd = (r2 * ri) * inv_del;
+  const float ri2 = ri * ri;
+  float H3 = ri2 * ri2 * ri;
+  if (!(rd > 8.0f)) {
+    const float rd2 = rd * rd;
+    const float E = expneg32(rd2);
+    const float ef = erf_tail32(rd, E);
+    const float p3 = v.q[7] + rd2 * (v.q[8] + rd2 * (v.q[9] + rd2 * v.q[10]));
+    H3 = (ef + p3 * (rd * E)) * H3;
+  }
+  const float gx = s.d0 - v.g0x, gy = s.d1 - v.g0y, gz = s.d2 - v.g0z;
+  const float k = H3 * (dx * s.nx + dy * s.ny + dz * s.nz) * (dx * gx + dy * gy + dz * gz);
+  u[0] += dx * k;
+  u[1] += dy * k;
+  u[2] += dz * k;
+}
+
 template <int KIND>
 __device__ __forceinline__ void load_target(const DirectArgs& A, int i, Tgt& t) {
   int ntp = A.ntp;
@@ -245,50 +366,84 @@ __device__ __forceinline__ void load_target(const DirectArgs& A, int i, Tgt& t) 
 
 // Sums over the sorted sources [j0, j1), which start at 64-subtile sub0.
 // Source records are read with wave-uniform addresses (scalar loads).
-template <int KIND, bool MIXED>
+//
+// Mixed precision (MIXED): a subtile beyond 8 delta of the wave runs the plain
+// kernel in FP32, on coordinates relative to the subtile's box centre (REL32,
+// the direct sums, whose FP32 records hold y - centre) or in FP64 (the leaf
+// sums).  In a mixed subtile a per-source wave vote sends the source to the
+// FP64 regularized code if some lane is within 4 delta, else to the FP32
+// regularized code if some lane is within 8 delta (d from the FP64
+// difference, so no cancellation), else to the FP64 plain kernel (which
+// reuses the vote's FP64 difference).  FP32 contributions are summed in FP32
+// over one subtile (at most 64 terms) and the subtile sums in FP64.
+template <int KIND, bool MIXED, bool REL32>
 __device__ __forceinline__ void range_sum(const DirectArgs& A, int j0, int j1, int sub0, const double* wb,
-                                          bool all_far, const Tgt& t, float pxf, float pyf, float pzf,
-                                          double u[3]) {
+                                          bool all_far, const Tgt& t, double u[3]) {
   const double rp = 1.7724538509055160273;
   const double inv_del = 1.0 / A.DEL;
   const double self1 = 32.0 / 5.0 / rp / A.DEL;
   const double self2 = 256.0 / 15.0 / rp / (A.DEL * A.DEL * A.DEL);
   const SrcRec* __restrict__ R = A.rec;
   const SrcRec32* __restrict__ R32 = A.rec32;
+  constexpr bool SL = (KIND == HOR_SL_ON || KIND == HOR_SL_OFF);
+  Tgt32 v;
+  if (MIXED) make_tgt32(t, v);
+  const float inv_del32 = (float)inv_del;
   for (int base = j0, sb = sub0; base < j1; base += kWave, sb++) {
     const int end = min(j1, base + kWave);
-    const bool fast = all_far || boxes_far(wb, A.sub_box + 6 * sb, A.far_d2);
+    const double* box = A.sub_box + 6 * sb;
+    const bool fast = all_far || boxes_far(wb, box, A.far_d2);
     if (fast) {
+      if (MIXED && REL32) {
+        const float cx = (float)(t.px - 0.5 * (box[0] + box[3]));
+        const float cy = (float)(t.py - 0.5 * (box[1] + box[4]));
+        const float cz = (float)(t.pz - 0.5 * (box[2] + box[5]));
+        float w[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll 2
-      for (int j = base; j < end; j++) {
-        if (KIND == HOR_SL_ON || KIND == HOR_SL_OFF)
-          sl_plain<MIXED>(R[j], MIXED ? R32[j] : SrcRec32{}, t.px, t.py, t.pz, pxf, pyf, pzf, t.sub, u);
-        else
-          dl_plain<MIXED>(R[j], MIXED ? R32[j] : SrcRec32{}, t.px, t.py, t.pz, pxf, pyf, pzf, t.g0x, t.g0y,
-                          t.g0z, u);
+        for (int j = base; j < end; j++) {
+          const SrcRec32& s = R32[j];
+          if (SL) sl_plain32(s, cx - s.x, cy - s.y, cz - s.z, v.sub, w);
+          else dl_plain32(s, cx - s.x, cy - s.y, cz - s.z, v.g0x, v.g0y, v.g0z, w);
+        }
+        u[0] += (double)w[0];
+        u[1] += (double)w[1];
+        u[2] += (double)w[2];
+      } else {
+#pragma unroll 2
+        for (int j = base; j < end; j++) {
+          if (SL) sl_plain(R[j], t.px, t.py, t.pz, t.sub, u);
+          else dl_plain(R[j], t.px, t.py, t.pz, t.g0x, t.g0y, t.g0z, u);
+        }
       }
     } else {
       // Mixed subtile: a wave vote per source decides whether any lane needs
       // the regularized kernel (r <= 8 delta, with a relative margin); when
       // none does, the source takes the plain path for the whole wave.
+      float w[3] = {0.0f, 0.0f, 0.0f};
       for (int j = base; j < end; j++) {
         const SrcRec s = R[j];
         const double dx = t.px - s.x, dy = t.py - s.y, dz = t.pz - s.z;
         const double r2 = dx * dx + dy * dy + dz * dz;
         const bool need = t.nearp && !(r2 > A.near_r2);
-        if (__any(need ? 1 : 0)) {
-          if (KIND == HOR_SL_ON || KIND == HOR_SL_OFF)
-            sl_full(s, t.px, t.py, t.pz, t.sub, t.nearp, t.q, inv_del, self1, self2, A.erfcx, u);
-          else if (KIND == HOR_DL_OFF)
-            dl_off_full(s, t, inv_del, self2, A.erfcx, u);
-          else
-            dl_on_full(s, t, inv_del, A.erfcx, u);
+        if (MIXED && !__any((t.nearp && !(r2 > A.near64_r2)) ? 1 : 0) && __any(need ? 1 : 0)) {
+          const SrcRec32& s32 = R32[j];
+          const float fx = (float)dx, fy = (float)dy, fz = (float)dz;
+          if (SL) sl_reg32(s32, fx, fy, fz, v, t.nearp, inv_del32, w);
+          else if (KIND == HOR_DL_OFF) dl_off_reg32(s32, fx, fy, fz, v, t.nearp, inv_del32, w);
+          else dl_on_reg32(s32, fx, fy, fz, v, inv_del32, w);
+        } else if (__any(need ? 1 : 0)) {
+          if (SL) sl_full(s, t.px, t.py, t.pz, t.sub, t.nearp, t.q, inv_del, self1, self2, A.erfcx, u);
+          else if (KIND == HOR_DL_OFF) dl_off_full(s, t, inv_del, self2, A.erfcx, u);
+          else dl_on_full(s, t, inv_del, A.erfcx, u);
         } else {
-          if (KIND == HOR_SL_ON || KIND == HOR_SL_OFF)
-            sl_plain<false>(s, SrcRec32{}, t.px, t.py, t.pz, pxf, pyf, pzf, t.sub, u);
-          else
-            dl_plain<false>(s, SrcRec32{}, t.px, t.py, t.pz, pxf, pyf, pzf, t.g0x, t.g0y, t.g0z, u);
+          if (SL) sl_plain(s, t.px, t.py, t.pz, t.sub, u);
+          else dl_plain(s, t.px, t.py, t.pz, t.g0x, t.g0y, t.g0z, u);
         }
+      }
+      if (MIXED) {
+        u[0] += (double)w[0];
+        u[1] += (double)w[1];
+        u[2] += (double)w[2];
       }
     }
   }

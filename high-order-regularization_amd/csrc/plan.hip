@@ -272,6 +272,7 @@ DirectArgs direct_args(hor_plan* P) {
   double fd = 8.0 * P->DEL * (1.0 + 1e-10);
   A.far_d2 = fd * fd;
   A.near_r2 = 64.0 * P->DEL * P->DEL * (1.0 + 1e-12);
+  A.near64_r2 = 16.0 * P->DEL * P->DEL;
   A.sx = P->sx.p; A.sn = P->sn.p; A.sa = P->sa.p; A.sd = P->sd.p; A.sub_box = P->sub_box.p;
   A.rec = P->rec.p; A.rec32 = P->precision == HOR_MIXED ? P->rec32.p : nullptr;
   A.tx = P->tx.p; A.tcoef = P->tcoef.p; A.tsub = P->tsub.p;
@@ -319,7 +320,7 @@ static int plan_eval_direct(hor_plan* P, const double* dens_dev, const double* d
   hor_ctx* c = P->ctx;
   hipStream_t s = c->stream;
   HOR_TRY(launch_gather3(dens_dev, P->sperm.p, P->ns, P->nsp, P->sx.p, P->sn.p, P->sa.p, P->sd.p, P->rec.p,
-                         P->precision == HOR_MIXED ? P->rec32.p : nullptr,
+                         P->precision == HOR_MIXED ? P->rec32.p : nullptr, P->sub_box.p,
                          P->kind == HOR_SL_ON || P->kind == HOR_SL_OFF, s));
   HOR_TRY(prepare_targets(P, dens_dev, d0_dev));
   DirectArgs A = direct_args(P);
