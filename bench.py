@@ -108,21 +108,23 @@ def cpu_baseline(s, t, DEL, target_seconds=15.0):
             "sample": f"{n} of {t['n']} band targets x {s['n']} sources, SL_OFF + DL_OFF, {dt:.1f}s"}
 
 
-def two_sphere_problem(h, gap, seed=SEED):
+def two_sphere_problem(h, gap, seed=SEED, ctx=None):
     """Two spheres of radius 0.5, centres (0,0,+-(1+gap)/2), the surface
     dicts of the two-surface solver (nearest data of each surface's points
     on the other surface) and two right-hand sides:
       rigid:   seeded rigid-body data b = U_k + W_k x (x - c_k), U, W ~ U(-1,1);
       forcing: the reference's b = -2/3 SL[f] with its surface force f
-               (main.cpp:199-207, surface.cpp:530-559)."""
+               (main.cpp:199-207, surface.cpp:530-559).
+    Quadrature and nearest data are built on the GPU of ctx when given, else
+    on the host."""
     from hor import geometry as G
     DEL = G.delta(h)
     zc = 0.5 + gap / 2
     bodies = [G.Body((0.5,) * 3, (0.0, 0.0, -zc)), G.Body((0.5,) * 3, (0.0, 0.0, zc))]
-    surfs = [b.surface(h) for b in bodies]
+    surfs = [b.surface(h, ctx=ctx) for b in bodies]
     pd = []
     for k in range(2):
-        t = bodies[1 - k].nearest(h, DEL, surfs[1 - k], surfs[k]["x"], with_force=True)
+        t = bodies[1 - k].nearest(h, DEL, surfs[1 - k], surfs[k]["x"], with_force=True, ctx=ctx)
         pd.append(dict(surfs[k], ctr=bodies[k].ctr, nrst=t["nrst"], x0=t["x0"], n0=t["n0"], b=t["b"],
                        i123=t["i123"], f=bodies[k].force(surfs[k]["x"], surfs[k]["nrml"]), f0=t["f0"]))
     rng = np.random.default_rng(seed)
@@ -132,6 +134,47 @@ def two_sphere_problem(h, gap, seed=SEED):
         x = pd[k]["x"].reshape(-1, 3)
         rigid.append((U + np.cross(Wv, x - bodies[k].ctr)).ravel())
     return DEL, bodies, pd, np.concatenate(rigid)
+
+
+def geometry_setup(ctx, h, gap, pd_gpu, gpu_s, sample=256):
+    """Geometry set-up on the GPU against the host: the same quadrature and
+    nearest data built by the host functions (time, and whether every
+    integer and position is identical), the GPU stencil build, and the
+    reference's stencil algorithm (the oracle's scan of every quadrature
+    point per target) timed on a sample of near targets and scaled to all."""
+    from hor import geometry as G
+    t0 = time.perf_counter()
+    _, _, pd_host, _ = two_sphere_problem(h, gap)
+    host_s = time.perf_counter() - t0
+    same = all(np.array_equal(pd_gpu[k][key], pd_host[k][key]) for k in range(2)
+               for key in ("x", "nrml", "plane", "nrst", "i123", "x0", "n0", "b", "f0"))
+    area = max(float(np.abs(pd_gpu[k]["area"] - pd_host[k]["area"]).max()) / h ** 2 for k in range(2))
+    srcs = [dict(pd_gpu[1], ctr=pd_gpu[1]["ctr"]), dict(pd_gpu[0], ctr=pd_gpu[0]["ctr"])]
+    G.stencils(ctx, srcs[0], pd_gpu[0], h)  # warm-up
+    t0 = time.perf_counter()
+    for k in range(2):
+        G.stencils(ctx, srcs[k], pd_gpu[k], h)
+    st_s = time.perf_counter() - t0
+    out = {"gpu_s": gpu_s, "host_s": host_s, "speedup": host_s / gpu_s, "identical_to_host": bool(same),
+           "area_max_abs_diff_over_h2": area, "stencil_gpu_s": st_s,
+           "note": "GPU: quadrature of both spheres + nearest data both ways (host arrays in and out); "
+                   "host: the same functions on the host cores (k-d tree nearest search)"}
+    from pyoracle import Oracle
+    O = Oracle()
+    near = [np.where(pd_gpu[k]["nrst"] != 0)[0] for k in range(2)]
+    tot = 0.0
+    for k in range(2):
+        pick = near[k][:sample]
+        sub = dict(n=len(pick), nrst=pd_gpu[k]["nrst"][pick], x0=pd_gpu[k]["x0"].reshape(-1, 3)[pick].ravel(),
+                   i123=pd_gpu[k]["i123"].reshape(-1, 3)[pick].ravel())
+        t0 = time.perf_counter()
+        O.stencil(srcs[k]["ctr"], h, srcs[k]["x"], sub)
+        tot += (time.perf_counter() - t0) * len(near[k]) / max(len(pick), 1)
+    out["stencil_cpu_baseline"] = {"seconds": tot, "kind": "port", "cores": 1,
+                                   "sample": f"{sample} near targets per surface, scaled to "
+                                             f"{len(near[0]) + len(near[1])} near targets (per matvec in the "
+                                             "reference, once per geometry here)"}
+    return out
 
 
 def cpu_matvec(h, DEL, pd, P, N0, theta, u, reps=1):
@@ -154,7 +197,7 @@ def cpu_matvec(h, DEL, pd, P, N0, theta, u, reps=1):
 
 
 def gmres_workload(h=1.0 / 128.0, gap=0.1, P=8, N0=4000, theta=0.6, tol=1e-8, seed=SEED,
-                   direct_check=False, cpu=False, cpu_reps=1):
+                   direct_check=False, cpu=False, cpu_reps=1, geometry=False):
     """Second-kind double-layer equation A u = u - 2/3 DL u = b on two spheres
     (two_sphere_problem), treecode matvec, GMRES(15) to relative residual tol,
     FP64 and mixed-precision treecodes, both right-hand sides.  Every solution's
@@ -162,15 +205,18 @@ def gmres_workload(h=1.0 / 128.0, gap=0.1, P=8, N0=4000, theta=0.6, tol=1e-8, se
     direct_check, with the FP64 direct-sum operator)."""
     import hor
     from hor.solver import Solver
-    t0 = time.time()
-    DEL, bodies, pd, b_rigid = two_sphere_problem(h, gap, seed)
-    geo_s = time.time() - t0
     ctx = hor.Context(0)
+    two_sphere_problem(1.0 / 16.0, gap, seed, ctx=ctx)  # first GPU geometry call (code object load)
+    t0 = time.perf_counter()
+    DEL, bodies, pd, b_rigid = two_sphere_problem(h, gap, seed, ctx=ctx)
+    geo_s = time.perf_counter() - t0
     tp = hor.TreeParams(P, N0, theta)
     n = b_rigid.size
     out = {"workload": f"two spheres r=0.5, gap {gap}, h=1/{round(1 / h)}, treecode P={P} N0={N0} "
                        f"theta={theta}, GMRES(15) to {tol:g}",
            "unknowns": int(n), "delta": DEL, "geometry_s": geo_s}
+    if geometry:
+        out["geometry"] = geometry_setup(ctx, h, gap, pd, geo_s)
     solvers = {}
     for name, prec in (("fp64", hor.FP64), ("mixed", hor.MIXED)):
         t0 = time.time()
@@ -290,7 +336,7 @@ def main():
             print(json.dumps(gmres_workload()), flush=True)
         else:
             print(json.dumps(gmres_workload(h=1.0 / args.gmres_only, P=10, N0=8000, cpu=args.cpu_h256,
-                                            cpu_reps=2)), flush=True)
+                                            cpu_reps=2, geometry=True)), flush=True)
         return
     body, s, t, DEL = build_workload(h=args.h)
     ctx = hor.Context(0)
@@ -382,7 +428,7 @@ def main():
         if not args.no_h256:
             log("two-sphere GMRES workload (h=1/256, treecode P=10 N0=8000)")
             gm["h256"] = gmres_workload(h=1.0 / 256.0, P=10, N0=8000, direct_check=True,
-                                        cpu=args.cpu_h256, cpu_reps=2)
+                                        cpu=args.cpu_h256, cpu_reps=2, geometry=not args.no_cpu)
     line = {
         "metric": "pair interactions/s",
         "value": value,

@@ -4,100 +4,23 @@
 // plans evaluated straight into the output vector, after the density at each
 // target's nearest point is interpolated on its 6x6 Monge stencil.  The
 // stencil indices and Lagrange weights depend on geometry only, so they are
-// found once on the host (the reference re-scans every quadrature point for
-// every target on every matvec, main.cpp:969-1006).  GMRES keeps its Krylov
-// basis in HBM; only the Hessenberg column and the rotations live on the host.
+// found once, on the device, at set-up (the reference re-scans every
+// quadrature point for every target on every matvec, main.cpp:969-1006).
+// GMRES keeps its Krylov basis in HBM; only the Hessenberg column and the
+// rotations live on the host.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <map>
 #include <memory>
-#include <unordered_map>
 #include <vector>
 
+#include "geom_gpu.h"
 #include "plan.h"
 
 using namespace hor;
 
 namespace hor {
-
-// sixPoint_coeff (main.cpp:750-757): 6-point Lagrange weights at offset p
-// for nodes -2..3.
-static void six_coeff(double p, double a[6]) {
-  a[0] = -p * (p * p - 1.0) * (p - 2.0) * (p - 3.0) / 120.0;
-  a[1] = p * (p - 1.0) * (p * p - 4.0) * (p - 3.0) / 24.0;
-  a[2] = -(p * p - 1.0) * (p * p - 4.0) * (p - 3.0) / 12.0;
-  a[3] = p * (p + 1.0) * (p * p - 4.0) * (p - 3.0) / 12.0;
-  a[4] = -p * (p * p - 1.0) * (p + 2.0) * (p - 3.0) / 24.0;
-  a[5] = p * (p * p - 1.0) * (p * p - 4.0) / 120.0;
-}
-
-// Stencil of every target (interpolate_DL_density + sixPoint_stencil,
-// main.cpp:682-748, 916-1046): slot [k1][k2] holds the last source point (in
-// index order) whose in-plane coordinates equal the slot's grid node within
-// 1e-14 and that lies on the same side (sign of the normal coordinate) as the
-// target's nearest point; 0 when no point matches (the reference's "unset").
-// Candidates are found through a hash of rounded grid coordinates instead of
-// the reference's full scan; the acceptance tests are the reference's.
-static void build_stencils(const double ctr[3], double h, int ns, const double* sx, int nt,
-                           const int* nrst, const double* x0, const int* i123, std::vector<int>& st,
-                           std::vector<double>& co) {
-  static const int pairs[3][2] = {{1, 2}, {0, 2}, {0, 1}};  // (i1, i2) for i3 = 0, 1, 2
-  std::unordered_map<uint64_t, std::vector<int>> bucket[3];
-  auto key = [](long l1, long l2) { return ((uint64_t)(uint32_t)(int32_t)l1 << 32) | (uint32_t)(int32_t)l2; };
-  for (int o = 0; o < 3; o++) {
-    int a = pairs[o][0], b = pairs[o][1];
-    for (int j = 0; j < ns; j++) {
-      long l1 = std::lround((sx[3 * j + a] - ctr[a]) / h);
-      long l2 = std::lround((sx[3 * j + b] - ctr[b]) / h);
-      bucket[o][key(l1, l2)].push_back(j);
-    }
-  }
-  st.assign(36 * (size_t)nt, 0);
-  co.assign(12 * (size_t)nt, 0.0);
-  const double tol = 1.e-14;
-#pragma omp parallel for schedule(dynamic, 64)
-  for (int i = 0; i < nt; i++) {
-    if (nrst[i] == 0) continue;
-    int ind1 = i123[3 * i], ind2 = i123[3 * i + 1], ind3 = i123[3 * i + 2];
-    const double* p0 = x0 + 3 * i;
-    double xLow = std::floor((p0[ind1] - ctr[ind1]) / h) * h + ctr[ind1];
-    double yLow = std::floor((p0[ind2] - ctr[ind2]) / h) * h + ctr[ind2];
-    double xHigh = xLow + h, yHigh = yLow + h;
-    double z0 = p0[ind3] - ctr[ind3];
-    double cx[6] = {xLow - 2.0 * h, xLow - h, xLow, xHigh, xHigh + h, xHigh + 2.0 * h};
-    double cy[6] = {yLow - 2.0 * h, yLow - h, yLow, yHigh, yHigh + h, yHigh + 2.0 * h};
-    // orientation whose bucket coordinates are (ind1, ind2)
-    int o = (ind3 == 0) ? 0 : (ind3 == 1) ? 1 : 2;
-    bool swapped = (pairs[o][0] != ind1);  // i123 always lists ind1 < ind2 for the reference patches
-    for (int c = 0; c < 6; c++)
-      for (int r = 0; r < 6; r++) {
-        long l1 = std::lround((cx[c] - ctr[ind1]) / h);
-        long l2 = std::lround((cy[r] - ctr[ind2]) / h);
-        auto it = bucket[o].find(swapped ? key(l2, l1) : key(l1, l2));
-        if (it == bucket[o].end()) continue;
-        int best = 0;
-        for (int j : it->second) {
-          double z = sx[3 * j + ind3] - ctr[ind3];
-          if (!(((z0 > 0) && (z > 0)) || ((z0 < 0) && (z < 0)))) continue;
-          double X = sx[3 * j + ind1], Y = sx[3 * j + ind2];
-          int col = -1, row = -1;
-          for (int q = 0; q < 6; q++)
-            if (std::fabs(cx[q] - X) < tol) { col = q; break; }
-          if (col != c) continue;
-          for (int q = 0; q < 6; q++)
-            if (std::fabs(cy[q] - Y) < tol) { row = q; break; }
-          if (row != r) continue;
-          best = std::max(best, j);
-        }
-        st[36 * (size_t)i + 6 * c + r] = best;
-      }
-    double p = (p0[ind1] - xLow) / h;
-    double q = (p0[ind2] - yLow) / h;
-    six_coeff(p, &co[12 * (size_t)i]);
-    six_coeff(q, &co[12 * (size_t)i + 6]);
-  }
-}
 
 // g0 = sum_j b_j sum_i a_i g[st[i][j]] (sixPointInterpolate, main.cpp:759-778).
 __global__ void k_interp(int nt, const int* nrst, const int* st, const double* co, const double* g,
@@ -301,11 +224,10 @@ extern "C" int hor_solver_create(hor_ctx* ctx, double h, double DEL, int method,
                             other->n, other->x, other->b, other->x0, other->n0, me->ctr, &S->tp[k],
                             &S->off[k]));
     // stencils of the points of surface k (their nearest points lie on the other surface)
-    std::vector<int> stv;
-    std::vector<double> cov;
-    build_stencils(other->ctr, h, other->n, other->x, me->n, me->nrst, me->x0, me->i123, stv, cov);
-    HOR_TRY(S->st[k].upload(stv, st));
-    HOR_TRY(S->co[k].upload(cov, st));
+    HOR_TRY(S->st[k].alloc(36 * (size_t)me->n));
+    HOR_TRY(S->co[k].alloc(12 * (size_t)me->n));
+    HOR_TRY(stencil_build(ctx, other->ctr, h, other->n, other->x, me->n, me->nrst, me->x0, me->i123, S->st[k].p,
+                          S->co[k].p));
     HOR_TRY(S->nrst[k].upload(me->nrst, me->n, st));
     HOR_TRY(S->g0[k].alloc(3 * (size_t)me->n));
   }

@@ -89,6 +89,9 @@ _SIGS = {
     "hor_nearest_data": (_I, [C.POINTER(Body), _D, _D, _I, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P]),
     "hor_surface_force": (_I, [C.POINTER(Body), _I, _P, _P, _P]),
     "hor_band_targets": (_I, [C.POINTER(Body), _P, _D, _I, _D, _I, _P, _I, _IP, _P, _P, _P, _P, _P]),
+    "hor_surface_generate_gpu": (_I, [_P, C.POINTER(Body), _D, _D, _I, _IP, _P, _P, _P, _P]),
+    "hor_nearest_data_gpu": (_I, [_P, C.POINTER(Body), _D, _D, _I, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P]),
+    "hor_stencil_build": (_I, [_P, _P, _D, _I, _P, _I, _P, _P, _P, _P, _P]),
     "hor_traction_translating": (_I, [C.POINTER(Body), _P, _I, _P, _P]),
     "hor_spheroid_exact_velocity": (_I, [C.POINTER(Body), _I, _P, _P]),
 }

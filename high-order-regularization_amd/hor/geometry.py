@@ -56,30 +56,37 @@ class Body:
         r = None if self.R is None else f64(self.R, 9)
         check(_lib.lib().hor_body_init(C.byref(self.c), ptr(a), ptr(c), ptr(r)), "hor_body_init")
 
-    def surface(self, h, pou_deg=70.0):
-        """Generate_Surface (surface.cpp:13-71), translated to the centre."""
+    def surface(self, h, pou_deg=70.0, ctx=None):
+        """Generate_Surface (surface.cpp:13-71), translated to the centre; on
+        the GPU of ctx when given (hor_surface_generate_gpu)."""
         L = _lib.lib()
+        if ctx is None:
+            fn, head, name = L.hor_surface_generate, (), "hor_surface_generate"
+        else:
+            fn, head, name = L.hor_surface_generate_gpu, (ctx.h,), "hor_surface_generate_gpu"
         n = C.c_int()
-        check(L.hor_surface_generate(C.byref(self.c), h, pou_deg, 0, C.byref(n), None, None, None, None),
-              "hor_surface_generate")
+        check(fn(*head, C.byref(self.c), h, pou_deg, 0, C.byref(n), None, None, None, None), name)
         n = n.value
         x, nr, a, pl = np.zeros(3 * n), np.zeros(3 * n), np.zeros(n), np.zeros(n, np.int32)
         m = C.c_int()
-        check(L.hor_surface_generate(C.byref(self.c), h, pou_deg, n, C.byref(m), ptr(x), ptr(nr), ptr(a),
-                                     ptr(pl)), "hor_surface_generate")
+        check(fn(*head, C.byref(self.c), h, pou_deg, n, C.byref(m), ptr(x), ptr(nr), ptr(a), ptr(pl)), name)
         return dict(n=n, x=x, nrml=nr, area=a, plane=pl, ctr=self.ctr.copy(), h=h)
 
-    def nearest(self, h, DEL, surf, tx, with_force=True):
-        """Nearest data of targets tx relative to this body's surface surf."""
+    def nearest(self, h, DEL, surf, tx, with_force=True, ctx=None):
+        """Nearest data of targets tx relative to this body's surface surf; on
+        the GPU of ctx when given (hor_nearest_data_gpu)."""
         tx = f64(tx)
         nt = tx.size // 3
         nrst = np.zeros(nt, np.int32)
         x0, n0, b = np.zeros(3 * nt), np.zeros(3 * nt), np.zeros(nt)
         i123 = np.zeros(3 * nt, np.int32)
         f0 = np.zeros(3 * nt) if with_force else None
-        check(_lib.lib().hor_nearest_data(C.byref(self.c), h, DEL, surf["n"], ptr(surf["x"]),
-                                          ptr(i32(surf["plane"])), nt, ptr(tx), ptr(nrst), ptr(x0),
-                                          ptr(n0), ptr(b), ptr(i123), ptr(f0)), "hor_nearest_data")
+        args = (C.byref(self.c), h, DEL, surf["n"], ptr(surf["x"]), ptr(i32(surf["plane"])), nt, ptr(tx),
+                ptr(nrst), ptr(x0), ptr(n0), ptr(b), ptr(i123), ptr(f0))
+        if ctx is None:
+            check(_lib.lib().hor_nearest_data(*args), "hor_nearest_data")
+        else:
+            check(_lib.lib().hor_nearest_data_gpu(ctx.h, *args), "hor_nearest_data_gpu")
         d = dict(n=nt, x=tx, nrst=nrst, x0=x0, n0=n0, b=b, i123=i123)
         if with_force:
             d["f0"] = f0
@@ -122,6 +129,19 @@ class Body:
         check(_lib.lib().hor_spheroid_exact_velocity(C.byref(self.c), x.size // 3, ptr(x), ptr(u)),
               "hor_spheroid_exact_velocity")
         return u
+
+
+def stencils(ctx, surf, tgt, h):
+    """6x6 Monge stencils of targets tgt (nearest data relative to surf) on
+    the GPU (hor_stencil_build): (nt, 36) source indices [k1][k2] and (nt, 12)
+    Lagrange weights along i1 then i2."""
+    nt = tgt["n"]
+    st, co = np.zeros(36 * nt, np.int32), np.zeros(12 * nt)
+    c = f64(np.asarray(surf["ctr"]), 3)
+    check(_lib.lib().hor_stencil_build(ctx.h, ptr(c), h, surf["n"], ptr(f64(surf["x"])), nt, ptr(i32(tgt["nrst"])),
+                                       ptr(f64(tgt["x0"])), ptr(i32(tgt["i123"])), ptr(st), ptr(co)),
+          "hor_stencil_build")
+    return st.reshape(nt, 36), co.reshape(nt, 12)
 
 
 # ---------------------------------------------------------------------------

@@ -338,6 +338,37 @@ int hor_band_targets(const hor_body* body, const double* origin, double h,
                      int* count, int* ijk, double* x, double* b, double* x0,
                      double* n0);
 
+/* ------------------------------------------------------------------------
+ * GPU geometry: the same functions on the device (one context's stream).
+ * Host pointers in and out.  Positions, normals, planes, nearest data and
+ * stencils are bit-identical to the host functions above; the quadrature
+ * weight may differ in its last bits (the partition-of-unity factor uses the
+ * device's exp and acos).
+ * ------------------------------------------------------------------------ */
+/* hor_surface_generate (Generate_Surface, surface.cpp:13-71) on the GPU. */
+int hor_surface_generate_gpu(hor_ctx* ctx, const hor_body* body, double h,
+                             double pou_deg, int cap, int* n, double* x,
+                             double* nrml, double* area, int* plane);
+
+/* hor_nearest_data (Compute_nearest_data, surface.cpp:450-526) on the GPU:
+ * the same nearest quadrature point (first index among equal distances),
+ * Newton projection, far sentinel and surface force. */
+int hor_nearest_data_gpu(hor_ctx* ctx, const hor_body* body, double h,
+                         double DEL, int ns, const double* sx, const int* splane,
+                         int nt, const double* tx, int* nrst, double* x0,
+                         double* n0, double* b, int* i123, double* f0);
+
+/* 6x6 Monge stencils of targets on a surface with centre ctr
+ * (interpolate_DL_density + sixPoint_stencil + sixPoint_coeff,
+ * main.cpp:682-757, 916-1046), as the solver builds them: stencil[36 nt]
+ * source indices [k1][k2] (0 where no point matches, the reference's unset
+ * value; all zero for far targets), coeff[12 nt] the Lagrange weights along
+ * i1 then i2. */
+int hor_stencil_build(hor_ctx* ctx, const double* ctr, double h, int ns,
+                      const double* sx, int nt, const int* nrst,
+                      const double* x0, const int* i123, int* stencil,
+                      double* coeff);
+
 /* Translating-body densities and exact velocity.
  * hor_traction_translating: single-layer density of a body translating with
  * unit velocity U (for an ellipsoid: F/(4 pi abc) / sqrt(x^2/a^4+...)), which
