@@ -29,8 +29,11 @@ SEED = 20261203
 PEAK_FP64 = 78.6e12          # MI355X FP64 (vector and matrix) FLOP/s
 # Algorithmic operation counts per pair (reference expressions, per-target
 # invariants hoisted, FMA = 2): plain SL 30, regularized off-surface SL 61,
-# plain DL 30, regularized off-surface DL 127.
-OPS = {"sl_plain": 30, "sl_reg": 61, "dl_plain": 30, "dl_reg": 127}
+# plain DL 30, regularized off-surface DL 127.  The pressure pair is counted in
+# its reduced form dx.w (w = -a f + (f0.n0) a n + (n0 x f0) x a n), fewer
+# operations than the reference's normal + tangential expression: plain 36,
+# regularized 63.
+OPS = {"sl_plain": 30, "sl_reg": 61, "dl_plain": 30, "dl_reg": 127, "p_plain": 36, "p_reg": 63}
 
 
 def log(*a):
@@ -106,6 +109,40 @@ def cpu_baseline(s, t, DEL, target_seconds=15.0):
     pairs = 2.0 * n * s["n"]
     return {"value": pairs / dt, "unit": "pair interactions/s", "cores": cores, "kind": kind,
             "sample": f"{n} of {t['n']} band targets x {s['n']} sources, SL_OFF + DL_OFF, {dt:.1f}s"}
+
+
+def pressure_cpu_baseline(s, t, DEL, target_seconds=6.0):
+    """The reference's stokes_pressure_7ord (compiled into oracle/_ref) on a
+    bounded target sample, all host cores; the oracle restatement when the
+    reference build is absent."""
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    try:
+        from pyoracle import RefGrid
+        impl, kind = RefGrid().pressure_7ord, "reference"
+    except (FileNotFoundError, OSError):
+        from pyoracle import Oracle
+        impl, kind = Oracle().pressure_off, "port"
+    rng = np.random.default_rng(3)
+
+    def sub(n):
+        idx = np.sort(rng.choice(t["n"], n, replace=False))
+        return dict(n=n, x=t["x"].reshape(-1, 3)[idx].ravel(), b=t["b"][idx],
+                    n0=t["n0"].reshape(-1, 3)[idx].ravel(), f0=t["f0"].reshape(-1, 3)[idx].ravel())
+
+    n0 = 500
+    t0 = time.perf_counter()
+    impl(DEL, s, sub(n0))
+    dt0 = time.perf_counter() - t0
+    n = int(min(t["n"], max(n0, n0 * target_seconds / max(dt0, 1e-3))))
+    tt = sub(n)
+    t0 = time.perf_counter()
+    impl(DEL, s, tt)
+    dt = time.perf_counter() - t0
+    return {"value": n * s["n"] / dt, "unit": "pair interactions/s", "cores": cores, "kind": kind,
+            "sample": f"{n} of {t['n']} band targets x {s['n']} sources, stokes_pressure_7ord, {dt:.1f}s"}
 
 
 def two_sphere_problem(h, gap, seed=SEED, ctx=None):
@@ -417,6 +454,27 @@ def main():
             "sl_achieved_tflops": sl_flops / (ksl_ms * 1e-3) / 1e12,
             "near_pair_fraction": fnear}
 
+    # regularized pressure (stokes_pressure_7ord) on the same band workload
+    pp = hor.Plan(ctx, hor.P_OFF, s, t, DEL)
+    out_p = ctx.alloc(8 * nt)
+    pp.eval_dev(dev["f"], dev["f0"], out_p)
+    kp = []
+    tp0 = time.perf_counter()
+    for _ in range(args.steps):
+        pp.eval_dev(dev["f"], dev["f0"], out_p)
+        kp.append(ctx.last_timing()[1])
+    p_ms = 1e3 * (time.perf_counter() - tp0) / args.steps
+    kp_ms = float(np.mean(kp))
+    p_flops = ns * nt * (fnear * OPS["p_reg"] + (1 - fnear) * OPS["p_plain"])
+    pressure = {"ms_per_eval": p_ms, "value": ns * nt / (p_ms * 1e-3), "unit": "pair interactions/s",
+                "kernel": "k_direct<P_OFF,FP64>", "avg_kernel_ms": kp_ms,
+                "achieved_tflops": p_flops / (kp_ms * 1e-3) / 1e12,
+                "frac": p_flops / (kp_ms * 1e-3) / PEAK_FP64}
+    pp.close()
+    ctx.free(out_p)
+    if not args.no_cpu:
+        pressure["cpu_baseline"] = pressure_cpu_baseline(s, t, DEL)
+
     cpu = None if args.no_cpu else cpu_baseline(s, t, DEL)
     gm = None
     if not args.no_gmres:
@@ -452,6 +510,7 @@ def main():
         "mixed": {"ms_per_step": mixed_ms, "value": pairs_per_step / (mixed_ms * 1e-3),
                   "rel_err_vs_fp64_sl": err_sl, "rel_err_vs_fp64_dl": err_dl},
         "accuracy": {"sl_max_err_inside_vs_exact_U": err_inside},
+        "pressure": pressure,
         "gmres": gm,
     }
     for p in dev.values():

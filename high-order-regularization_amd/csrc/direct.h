@@ -26,7 +26,7 @@ struct DirectArgs {
   const double* sub_box;           // 6 per 64-source subtile (lo xyz, hi xyz)
   const double* tx;                // 3*ntp positions
   const double* tcoef;             // 12*ntp regularization polynomial coefficients
-  const double* tsub;              // SL: ntp f0.n0; DL: 3*ntp g0
+  const double* tsub;              // SL: ntp f0.n0; DL: 3*ntp g0; P: ntp f0.n0, 3*ntp n0 x f0
   const double* tx0;               // DL_OFF: 3*ntp nearest point
   const double* tn0;               // DL_OFF: 3*ntp nearest normal
   const double* tb;                // DL_OFF: ntp signed distance
@@ -40,6 +40,9 @@ int launch_direct(const DirectArgs& A, int kind, bool mixed, int nchunk, hipStre
 int launch_finalize(const double* partial, int nchunk, int nt, int ntp, double scale,
                     const double* xi, const double* g0, const int* tperm, double* out,
                     const double* base, double coef, int accumulate, hipStream_t s);
+int launch_pressure_finalize(const double* partial, int nchunk, int nt, int ntp, double scale, const double* xi,
+                             const double* f0n0, const int* tperm, double* out, const double* base, double coef,
+                             int accumulate, hipStream_t s);
 int launch_gather3(const double* dens, const int* perm, int n, int np, const double* sx, const double* sn,
                    const double* sa, double* sd, struct SrcRec* rec, struct SrcRec32* rec32,
                    const double* sub_box, int scale_density, hipStream_t s);

@@ -36,8 +36,27 @@ __global__ __launch_bounds__(kBlock) void k_direct(DirectArgs A) {
   range_sum<KIND, MIXED, true>(A, sub0 * kWave, min(A.ns, sub1 * kWave), sub0, wb, all_far, t, u);
   double* P = A.partial + (size_t)blockIdx.y * 3 * A.ntp;
   P[i] = u[0];
-  P[A.ntp + i] = u[1];
-  P[2 * A.ntp + i] = u[2];
+  if (KIND != HOR_P_OFF) {
+    P[A.ntp + i] = u[1];
+    P[2 * A.ntp + i] = u[2];
+  }
+}
+
+// Pressure: one value per target, v = scale*sum + xi*(f0.n0) with xi = -1
+// inside (b < -1e-12, stokes.cpp:198-200), combined into out like
+// k_direct_finalize.
+__global__ void k_pressure_finalize(const double* partial, int nchunk, int nt, int ntp, double scale,
+                                    const double* xi, const double* f0n0, const int* tperm, double* out,
+                                    const double* base, double coef, int accumulate) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nt) return;
+  int o = tperm[i];
+  double s = 0.0;
+  for (int c = 0; c < nchunk; c++) s += partial[(size_t)c * 3 * ntp + i];
+  double w = coef * (s * scale + xi[i] * f0n0[i]);
+  if (accumulate) out[o] += w;
+  else if (base) out[o] = base[o] + w;
+  else out[o] = w;
 }
 
 // Adds the chunk partials in chunk order, scales, adds the jump term and
@@ -107,6 +126,7 @@ __global__ void k_gather3(const double* dens, const int* perm, int n, int np, co
 //   mode 2 (DL_OFF): g0 = near ? d0 : 0            (leaf / direct sums)
 //   mode 3 (DL_ON):  g0 = g_o of source o          (stresslets.cpp:212-214)
 //   mode 4 (DL_OFF): g0 = d0 for every target      (treecode proxies and jump)
+//   mode 5 (P_OFF):  [d0.n0, n0 x d0] for every target (stokes.cpp:146-147)
 __global__ void k_target_sub(int mode, int nt, int ntp, const int* tperm, const int* tnear,
                              const double* d0, const double* n0_orig, const double* dens,
                              const double* sn_orig, double* out) {
@@ -115,6 +135,7 @@ __global__ void k_target_sub(int mode, int nt, int ntp, const int* tperm, const 
   if (i >= nt) {
     out[i] = 0;
     if (mode >= 2) { out[ntp + i] = 0; out[2 * ntp + i] = 0; }
+    if (mode == 5) out[3 * ntp + i] = 0;
     return;
   }
   int o = tperm[i];
@@ -130,6 +151,13 @@ __global__ void k_target_sub(int mode, int nt, int ntp, const int* tperm, const 
     for (int k = 0; k < 3; k++) out[(size_t)k * ntp + i] = nearp ? d0[3 * o + k] : 0.0;
   } else if (mode == 3) {
     for (int k = 0; k < 3; k++) out[(size_t)k * ntp + i] = dens[3 * o + k];
+  } else if (mode == 5) {
+    const double* f = d0 + 3 * o;
+    const double* n = n0_orig + 3 * o;
+    out[i] = f[0] * n[0] + f[1] * n[1] + f[2] * n[2];
+    out[ntp + i] = n[1] * f[2] - n[2] * f[1];
+    out[2 * (size_t)ntp + i] = n[2] * f[0] - n[0] * f[2];
+    out[3 * (size_t)ntp + i] = n[0] * f[1] - n[1] * f[0];
   } else {
     for (int k = 0; k < 3; k++) out[(size_t)k * ntp + i] = d0[3 * o + k];
   }
@@ -156,9 +184,11 @@ int launch_direct(const DirectArgs& A, int kind, bool mixed, int nchunk, hipStre
       case HOR_SL_ON: HOR_LAUNCH(HOR_SL_ON, false); break;
       case HOR_SL_OFF: HOR_LAUNCH(HOR_SL_OFF, false); break;
       case HOR_DL_ON: HOR_LAUNCH(HOR_DL_ON, false); break;
+      case HOR_P_OFF: HOR_LAUNCH(HOR_P_OFF, false); break;
       default: HOR_LAUNCH(HOR_DL_OFF, false); break;
     }
   } else {
+    if (kind == HOR_P_OFF) return set_error(HOR_E_ARG, "the pressure operator is FP64 only");
     switch (kind) {
       case HOR_SL_ON: HOR_LAUNCH(HOR_SL_ON, true); break;
       case HOR_SL_OFF: HOR_LAUNCH(HOR_SL_OFF, true); break;
@@ -177,6 +207,16 @@ int launch_finalize(const double* partial, int nchunk, int nt, int ntp, double s
   if (nt == 0) return HOR_OK;
   hipLaunchKernelGGL(k_direct_finalize, dim3((nt + 255) / 256), dim3(256), 0, s, partial, nchunk, nt,
                      ntp, scale, xi, g0, tperm, out, base, coef, accumulate);
+  HOR_HIP_TRY(hipGetLastError());
+  return HOR_OK;
+}
+
+int launch_pressure_finalize(const double* partial, int nchunk, int nt, int ntp, double scale, const double* xi,
+                             const double* f0n0, const int* tperm, double* out, const double* base, double coef,
+                             int accumulate, hipStream_t s) {
+  if (nt == 0) return HOR_OK;
+  hipLaunchKernelGGL(k_pressure_finalize, dim3((nt + 255) / 256), dim3(256), 0, s, partial, nchunk, nt, ntp, scale,
+                     xi, f0n0, tperm, out, base, coef, accumulate);
   HOR_HIP_TRY(hipGetLastError());
   return HOR_OK;
 }

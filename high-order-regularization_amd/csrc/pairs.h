@@ -338,6 +338,54 @@ __device__ __forceinline__ void dl_on_reg32(const SrcRec32& s, float dx, float d
   u[2] += dz * k;
 }
 
+// ---- Pressure pair (stokes_pressure_7ord, extend_to_grid/stokes.cpp:134-203).
+// The reference's summand [-(f.n - f0.n0)(n.dx) - (n x f - n0 x f0).(n x dx)] A
+// equals dx.w with w = -(a f) + (f0.n0)(a n) + (n0 x f0) x (a n) for a unit
+// normal n (Binet-Cauchy and the triple product), so with the record's a f
+// and a n every pair costs one weight vector and one dot product.  Tgt.sub
+// holds f0.n0 and Tgt.g0 holds n0 x f0.
+__device__ __forceinline__ double p_numer(const SrcRec& s, const Tgt& t, double dx, double dy, double dz) {
+  const double wx = t.sub * s.nx - s.d0 + (t.g0y * s.nz - t.g0z * s.ny);
+  const double wy = t.sub * s.ny - s.d1 + (t.g0z * s.nx - t.g0x * s.nz);
+  const double wz = t.sub * s.nz - s.d2 + (t.g0x * s.ny - t.g0y * s.nx);
+  return wx * dx + wy * dy + wz * dz;
+}
+
+// Plain kernel H2 = 1/r^3 (Lapl_DL_7ord beyond 8 delta).
+__device__ __forceinline__ void p_plain(const SrcRec& s, const Tgt& t, double u[3]) {
+  const double dx = t.px - s.x, dy = t.py - s.y, dz = t.pz - s.z;
+  const double r2 = dx * dx + dy * dy + dz * dz;
+  const double ri = rsqrt64(r2);
+  u[0] += p_numer(s, t, dx, dy, dz) * (ri * ri * ri);
+}
+
+// Full branch structure of Lapl_DL_7ord (extend_to_grid/utilities.cpp:190-211):
+// self limit, plain, or (erf + P2(rho^2) rho e^{-rho^2}) / r^3 with the
+// Stokes_SL_7ord s2 polynomial (the same coefficients q[3..6]).
+__device__ __forceinline__ void p_full(const SrcRec& s, const Tgt& t, double inv_del, double self2,
+                                       const double* ec, double u[3]) {
+  const double dx = t.px - s.x, dy = t.py - s.y, dz = t.pz - s.z;
+  const double r2 = dx * dx + dy * dy + dz * dz;
+  double H2;
+  if (t.nearp && r2 < 1e-28) {
+    H2 = self2;
+  } else {
+    const double ri = rsqrt64(r2);
+    const double rd = (r2 * ri) * inv_del;
+    const double ri3 = ri * ri * ri;
+    if (!t.nearp || rd > 8.0) {
+      H2 = ri3;
+    } else {
+      const double rd2 = rd * rd;
+      const double E = exp(-rd2);
+      const double ef = erf_from_exp(rd, E, ec);
+      const double p2 = t.q[3] + rd2 * (t.q[4] + rd2 * (t.q[5] + rd2 * t.q[6]));
+      H2 = (ef + p2 * (rd * E)) * ri3;
+    }
+  }
+  u[0] += p_numer(s, t, dx, dy, dz) * H2;
+}
+
 template <int KIND>
 __device__ __forceinline__ void load_target(const DirectArgs& A, int i, Tgt& t) {
   int ntp = A.ntp;
@@ -352,6 +400,11 @@ __device__ __forceinline__ void load_target(const DirectArgs& A, int i, Tgt& t) 
   t.x0x = t.x0y = t.x0z = t.n0x = t.n0y = t.n0z = t.b = 0;
   if (KIND == HOR_SL_ON || KIND == HOR_SL_OFF) {
     t.sub = A.tsub[i];
+  } else if (KIND == HOR_P_OFF) {
+    t.sub = A.tsub[i];             // f0.n0
+    t.g0x = A.tsub[ntp + i];       // n0 x f0
+    t.g0y = A.tsub[2 * ntp + i];
+    t.g0z = A.tsub[3 * ntp + i];
   } else {
     t.g0x = A.tsub[i];
     t.g0y = A.tsub[ntp + i];
@@ -386,6 +439,7 @@ __device__ __forceinline__ void range_sum(const DirectArgs& A, int j0, int j1, i
   const SrcRec* __restrict__ R = A.rec;
   const SrcRec32* __restrict__ R32 = A.rec32;
   constexpr bool SL = (KIND == HOR_SL_ON || KIND == HOR_SL_OFF);
+  constexpr bool PR = (KIND == HOR_P_OFF);   // FP64 only
   Tgt32 v;
   if (MIXED) make_tgt32(t, v);
   const float inv_del32 = (float)inv_del;
@@ -411,7 +465,8 @@ __device__ __forceinline__ void range_sum(const DirectArgs& A, int j0, int j1, i
       } else {
 #pragma unroll 2
         for (int j = base; j < end; j++) {
-          if (SL) sl_plain(R[j], t.px, t.py, t.pz, t.sub, u);
+          if (PR) p_plain(R[j], t, u);
+          else if (SL) sl_plain(R[j], t.px, t.py, t.pz, t.sub, u);
           else dl_plain(R[j], t.px, t.py, t.pz, t.g0x, t.g0y, t.g0z, u);
         }
       }
@@ -432,11 +487,13 @@ __device__ __forceinline__ void range_sum(const DirectArgs& A, int j0, int j1, i
           else if (KIND == HOR_DL_OFF) dl_off_reg32(s32, fx, fy, fz, v, t.nearp, inv_del32, w);
           else dl_on_reg32(s32, fx, fy, fz, v, inv_del32, w);
         } else if (__any(need ? 1 : 0)) {
-          if (SL) sl_full(s, t.px, t.py, t.pz, t.sub, t.nearp, t.q, inv_del, self1, self2, A.erfcx, u);
+          if (PR) p_full(s, t, inv_del, self2, A.erfcx, u);
+          else if (SL) sl_full(s, t.px, t.py, t.pz, t.sub, t.nearp, t.q, inv_del, self1, self2, A.erfcx, u);
           else if (KIND == HOR_DL_OFF) dl_off_full(s, t, inv_del, self2, A.erfcx, u);
           else dl_on_full(s, t, inv_del, A.erfcx, u);
         } else {
-          if (SL) sl_plain(s, t.px, t.py, t.pz, t.sub, u);
+          if (PR) p_plain(s, t, u);
+          else if (SL) sl_plain(s, t.px, t.py, t.pz, t.sub, u);
           else dl_plain(s, t.px, t.py, t.pz, t.g0x, t.g0y, t.g0z, u);
         }
       }

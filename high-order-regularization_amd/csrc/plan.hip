@@ -114,7 +114,7 @@ bool target_is_near(const hor_plan* P, double b) {
   int kind = P->kind;
   double DEL = P->DEL;
   if (kind == HOR_SL_ON || kind == HOR_DL_ON || P->all_near) return true;
-  if (kind == HOR_SL_OFF && P->method == HOR_DIRECT) return !(std::fabs(b / DEL) > 8.0);
+  if ((kind == HOR_SL_OFF || kind == HOR_P_OFF) && P->method == HOR_DIRECT) return !(std::fabs(b / DEL) > 8.0);
   return !(std::fabs(b) / DEL > 8.0);  // stresslets.cpp:41, KITC.cpp:856, 970
 }
 
@@ -181,13 +181,14 @@ int setup_targets(hor_plan* P, const double* tx, const double* tb, const double*
       else
         XI[i] = (std::fabs(b) < 1.0e-12) ? 0.5 : ((b < 0.0) ? 1.0 : 0.0);   // KITC.cpp:1805-1811
     }
+    if (kind == HOR_P_OFF) XI[i] = (b < -1.e-12) ? -1.0 : 0.0;              // stokes.cpp:198-200
   }
   // wave boxes
   int nw = ntp / kWave;
   std::vector<double> WB(8 * (size_t)nw);
   for (int w = 0; w < nw; w++) {
     double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-    bool all_far = (kind == HOR_SL_OFF || kind == HOR_DL_OFF);
+    bool all_far = (kind == HOR_SL_OFF || kind == HOR_DL_OFF || kind == HOR_P_OFF);
     for (int i = w * kWave; i < (w + 1) * kWave; i++) {
       for (int k = 0; k < 3; k++) {
         double v = X[(size_t)k * ntp + i];
@@ -211,8 +212,9 @@ int setup_targets(hor_plan* P, const double* tx, const double* tb, const double*
     HOR_TRY(P->tn0.upload(N0, s));
     HOR_TRY(P->tb.upload(Bv, s));
   }
-  if (kind == HOR_SL_OFF || kind == HOR_DL_OFF) HOR_TRY(P->n0_orig.upload(tn0, 3 * (size_t)nt, s));
-  HOR_TRY(P->tsub.alloc(3 * (size_t)ntp));
+  if (kind == HOR_SL_OFF || kind == HOR_DL_OFF || kind == HOR_P_OFF)
+    HOR_TRY(P->n0_orig.upload(tn0, 3 * (size_t)nt, s));
+  HOR_TRY(P->tsub.alloc((kind == HOR_P_OFF ? 4 : 3) * (size_t)ntp));
   HOR_TRY(P->tsub2.alloc(3 * (size_t)ntp));
   HOR_TRY(P->in_d0.alloc(3 * (size_t)std::max(nt, 1)));
   HOR_TRY(P->out_buf.alloc(3 * (size_t)std::max(nt, 1)));
@@ -284,6 +286,7 @@ DirectArgs direct_args(hor_plan* P) {
 
 double op_scale(int kind) {
   const double denom = 1.0 / (8.0 * kPi);                      // utilities.h:11
+  if (kind == HOR_P_OFF) return -2.0 * denom;                   // -1/(4 pi), stokes.cpp:196
   return (kind == HOR_SL_ON || kind == HOR_SL_OFF) ? denom : -6.0 * denom;
 }
 
@@ -291,6 +294,7 @@ double op_scale(int kind) {
 int prepare_targets(hor_plan* P, const double* dens_dev, const double* d0_dev) {
   hipStream_t s = P->ctx->stream;
   int mode = (P->kind == HOR_SL_OFF) ? 0 : (P->kind == HOR_SL_ON) ? 1 : (P->kind == HOR_DL_OFF) ? 2 : 3;
+  if (P->kind == HOR_P_OFF) mode = 5;
   if (P->on_at) mode = 4;  // on-surface stresslet at given targets: g0 = d0
   HOR_TRY(launch_target_sub(mode, P->nt, P->ntp, P->tperm.p, P->tnear.p, d0_dev, P->n0_orig.p, dens_dev,
                             P->sn_orig.p, P->tsub.p, s));
@@ -321,12 +325,15 @@ static int plan_eval_direct(hor_plan* P, const double* dens_dev, const double* d
   hipStream_t s = c->stream;
   HOR_TRY(launch_gather3(dens_dev, P->sperm.p, P->ns, P->nsp, P->sx.p, P->sn.p, P->sa.p, P->sd.p, P->rec.p,
                          P->precision == HOR_MIXED ? P->rec32.p : nullptr, P->sub_box.p,
-                         P->kind == HOR_SL_ON || P->kind == HOR_SL_OFF, s));
+                         P->kind == HOR_SL_ON || P->kind == HOR_SL_OFF || P->kind == HOR_P_OFF, s));
   HOR_TRY(prepare_targets(P, dens_dev, d0_dev));
   DirectArgs A = direct_args(P);
   HOR_HIP_TRY(hipEventRecord(c->ev[2], s));
   HOR_TRY(launch_direct(A, P->kind, P->precision == HOR_MIXED, P->nchunk, s));
   HOR_HIP_TRY(hipEventRecord(c->ev[3], s));
+  if (P->kind == HOR_P_OFF)
+    return launch_pressure_finalize(P->partial.p, P->nchunk, P->nt, P->ntp, op_scale(P->kind), P->xi.p,
+                                    P->tsub.p, P->tperm.p, out_dev, base, coef, accumulate, s);
   bool dl = (P->kind == HOR_DL_ON || P->kind == HOR_DL_OFF);
   HOR_TRY(launch_finalize(P->partial.p, P->nchunk, P->nt, P->ntp, op_scale(P->kind), dl ? P->xi.p : nullptr,
                           dl ? P->tsub.p : nullptr, P->tperm.p, out_dev, base, coef, accumulate, s));
@@ -351,7 +358,9 @@ extern "C" int hor_plan_create(hor_ctx* ctx, int kind, int method, int precision
   *plan = nullptr;
   const int flags = kind & 0xF0;
   kind &= 0x0F;
-  if (kind < HOR_SL_ON || kind > HOR_DL_OFF) return set_error(HOR_E_ARG, "bad kind %d", kind);
+  if (kind < HOR_SL_ON || kind > HOR_P_OFF) return set_error(HOR_E_ARG, "bad kind %d", kind);
+  if (kind == HOR_P_OFF && (flags || method != HOR_DIRECT || precision != HOR_FP64))
+    return set_error(HOR_E_ARG, "the pressure operator is a direct FP64 sum without kind flags");
   if ((flags & ~(HOR_ALL_NEAR | HOR_ON_AT)) || ((flags & HOR_ALL_NEAR) && kind != HOR_SL_OFF && kind != HOR_DL_OFF) ||
       ((flags & HOR_ON_AT) && kind != HOR_DL_ON))
     return set_error(HOR_E_ARG, "bad kind flags 0x%x for kind %d", flags, kind);
@@ -431,9 +440,10 @@ extern "C" int hor_plan_eval(hor_plan* P, const double* dens, const double* d0, 
   HOR_HIP_TRY(hipMemcpyAsync(P->in_dens.p, dens, 3 * sizeof(double) * P->ns, hipMemcpyHostToDevice, s));
   if (!on) HOR_HIP_TRY(hipMemcpyAsync(P->in_d0.p, d0, 3 * sizeof(double) * P->nt, hipMemcpyHostToDevice, s));
   HOR_TRY(hor_plan_eval_dev(P, P->in_dens.p, on ? nullptr : P->in_d0.p, P->out_buf.p));
-  HOR_HIP_TRY(hipMemcpyAsync(out, P->out_buf.p, 3 * sizeof(double) * P->nt, hipMemcpyDeviceToHost, s));
+  const size_t nout = (P->kind == HOR_P_OFF ? 1 : 3) * (size_t)P->nt;
+  HOR_HIP_TRY(hipMemcpyAsync(out, P->out_buf.p, sizeof(double) * nout, hipMemcpyDeviceToHost, s));
   HOR_HIP_TRY(hipStreamSynchronize(s));
-  if (!all_finite(out, 3 * (size_t)P->nt)) return set_error(HOR_E_NAN, "non-finite result");
+  if (!all_finite(out, nout)) return set_error(HOR_E_NAN, "non-finite result");
   return HOR_OK;
 }
 
@@ -510,4 +520,23 @@ extern "C" int hor_dl_offsurf_tree(hor_ctx* ctx, const double* ctr, const hor_tr
                                    const double* tx0, const double* tn0, const double* tg0, double* out) {
   return one_shot(ctx, HOR_DL_OFF, HOR_TREE, ctr, tree, DEL, ns, sx, sn, sa, sg, nt, tx, tb, tx0, tn0, tg0,
                   out);
+}
+
+// stokes_pressure_7ord (extend_to_grid/stokes.cpp:134-203) at every target.
+extern "C" int hor_pressure_7ord(hor_ctx* ctx, double DEL, int ns, const double* sx, const double* sn,
+                                 const double* sa, const double* sf, int nt, const double* tx, const double* tb,
+                                 const double* tn0, const double* tf0, double* out) {
+  return one_shot(ctx, HOR_P_OFF, HOR_DIRECT, nullptr, nullptr, DEL, ns, sx, sn, sa, sf, nt, tx, tb, nullptr,
+                  tn0, tf0, out);
+}
+
+// stokes_pressure_away (extend_to_grid/stokes.cpp:102-127): the pressure
+// operator with every target beyond 8 delta (plain 1/r^3 for every pair) and
+// no subtraction (f0 = 0), which is 1/(4 pi) sum A f.dx / r^3.
+extern "C" int hor_pressure_away(hor_ctx* ctx, int ns, const double* sx, const double* sn, const double* sa,
+                                 const double* sf, int nt, const double* tx, double* out) {
+  if (nt <= 0) return set_error(HOR_E_ARG, "hor_pressure_away: need nt > 0");
+  std::vector<double> far_b((size_t)nt, 1e300), zero(3 * (size_t)nt, 0.0);
+  return one_shot(ctx, HOR_P_OFF, HOR_DIRECT, nullptr, nullptr, 1.0, ns, sx, sn, sa, sf, nt, tx, far_b.data(),
+                  nullptr, zero.data(), zero.data(), out);
 }

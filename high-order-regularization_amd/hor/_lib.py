@@ -14,7 +14,7 @@ _IP = C.POINTER(C.c_int)
 _DP = C.POINTER(C.c_double)
 
 HOR_OK = 0
-SL_ON, SL_OFF, DL_ON, DL_OFF = 0, 1, 2, 3
+SL_ON, SL_OFF, DL_ON, DL_OFF, P_OFF = 0, 1, 2, 3, 4
 DIRECT, TREE = 0, 1
 FP64, MIXED = 0, 1
 
@@ -56,6 +56,8 @@ _SIGS = {
     "hor_dl_onsurf_tree": (_I, [_P, _P, C.POINTER(TreeParams), _D, _I, _P, _P, _P, _P, _I, _P, _P]),
     "hor_dl_offsurf_tree": (_I, [_P, _P, C.POINTER(TreeParams), _D, _I, _P, _P, _P, _P, _I, _P,
                                  _P, _P, _P, _P, _P]),
+    "hor_pressure_7ord": (_I, [_P, _D, _I, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P]),
+    "hor_pressure_away": (_I, [_P, _I, _P, _P, _P, _P, _I, _P, _P]),
     "hor_solver_create": (_I, [_P, _D, _D, _I, _I, C.POINTER(TreeParams), C.POINTER(TreeParams),
                                C.POINTER(SurfaceDesc), C.POINTER(SurfaceDesc), C.POINTER(_P)]),
     "hor_solver_destroy": (_I, [_P]),

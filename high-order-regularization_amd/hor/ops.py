@@ -1,8 +1,9 @@
 """Operator API over libhor.so: context, plans, and the reference's functions.
 
 The module-level functions carry the reference's names and argument meaning
-(stokeslets.h:9-29, stresslets.h:9-30, KITC.h:99-118) with surfaces/targets as
-dicts (see hor.geometry).  Everything runs on the GPU; a missing library or
+(stokeslets.h:9-29, stresslets.h:9-30, KITC.h:99-118, and the grid program's
+stokes.h:20-28 for the pressure) with surfaces/targets as dicts (see
+hor.geometry).  Everything runs on the GPU; a missing library or
 device raises instead of falling back to the CPU.
 """
 from __future__ import annotations
@@ -13,13 +14,13 @@ import weakref
 import numpy as np
 
 from . import _lib
-from ._lib import DIRECT, DL_OFF, DL_ON, FP64, MIXED, SL_OFF, SL_ON, TREE, check, f64, ptr
+from ._lib import DIRECT, DL_OFF, DL_ON, FP64, MIXED, P_OFF, SL_OFF, SL_ON, TREE, check, f64, ptr
 
-__all__ = ["Context", "Plan", "TreeParams", "SL_ON", "SL_OFF", "DL_ON", "DL_OFF", "DIRECT", "TREE",
+__all__ = ["Context", "Plan", "TreeParams", "SL_ON", "SL_OFF", "DL_ON", "DL_OFF", "P_OFF", "DIRECT", "TREE",
            "FP64", "MIXED", "eval_Stokes_SL_offSurf", "eval_Stokes_SL_onSurf",
            "eval_Stokes_DL_offSurf", "eval_Stokes_DL_onSurf", "eval_Stokes_SL_onSurf_tree",
            "eval_Stokes_SL_offSurf_tree", "eval_Stokes_DL_onSurf_tree", "eval_Stokes_DL_offSurf_tree",
-           "device_count"]
+           "stokes_pressure_7ord", "stokes_pressure_away", "device_count"]
 
 
 def device_count():
@@ -136,7 +137,7 @@ class Plan:
     def __call__(self, dens, d0=None):
         dens = f64(dens, 3 * self.ns)
         d0 = None if d0 is None else f64(d0, 3 * self.nt)
-        out = np.zeros(3 * self.nt)
+        out = np.zeros((1 if self.kind == P_OFF else 3) * self.nt)
         check(_lib.lib().hor_plan_eval(self.h, ptr(dens), ptr(d0), ptr(out)), "hor_plan_eval")
         return out
 
@@ -191,3 +192,22 @@ def eval_Stokes_DL_onSurf_tree(ctx, ctr, h, DEL, surf, targets=None, tree=None):
 def eval_Stokes_DL_offSurf_tree(ctx, ctr, h, DEL, surf, targets, tree=None):
     """KITC.cpp:1712-1830."""
     return Plan(ctx, DL_OFF, surf, targets, DEL, TREE, ctr=ctr, tree=tree or TreeParams())(surf["g"], targets["g0"])
+
+
+def stokes_pressure_7ord(ctx, DEL, surf, targets):
+    """stokes_pressure_7ord (extend_to_grid/stokes.cpp:134-203) at every target:
+    surf['f'] the surface force, targets['f0'] the force at the nearest point;
+    one pressure per target."""
+    return Plan(ctx, P_OFF, surf, targets, DEL)(surf["f"], targets["f0"])
+
+
+def stokes_pressure_away(ctx, surf, tx):
+    """stokes_pressure_away (extend_to_grid/stokes.cpp:102-127): plain sum at
+    targets away from the surface."""
+    tx = f64(tx)
+    nt = tx.size // 3
+    out = np.zeros(nt)
+    f, x, nr, a = f64(surf["f"]), f64(surf["x"]), f64(surf["nrml"]), f64(surf["area"])
+    check(_lib.lib().hor_pressure_away(ctx.h, surf["n"], ptr(x), ptr(nr), ptr(a), ptr(f), nt, ptr(tx), ptr(out)),
+          "hor_pressure_away")
+    return out

@@ -43,7 +43,10 @@ enum {
   HOR_SL_ON = 0,   /* eval_Stokes_SL_onSurf   (stokeslets.cpp:141-180) */
   HOR_SL_OFF = 1,  /* eval_Stokes_SL_offSurf  (stokeslets.cpp:12-86)   */
   HOR_DL_ON = 2,   /* eval_Stokes_DL_onSurf   (stresslets.cpp:186-228) */
-  HOR_DL_OFF = 3   /* eval_Stokes_DL_offSurf  (stresslets.cpp:13-96)   */
+  HOR_DL_OFF = 3,  /* eval_Stokes_DL_offSurf  (stresslets.cpp:13-96)   */
+  HOR_P_OFF = 4    /* stokes_pressure_7ord (extend_to_grid/stokes.cpp:134-203):
+                      regularized pressure of the single layer with subtraction;
+                      one scalar per target; direct method, FP64 only */
 };
 
 /* Modifier bits OR-ed into the kind of hor_plan_create (the reference's
@@ -102,6 +105,7 @@ typedef struct {
  *       as the reference assumes: stokeslets.cpp:167, KITC.cpp:1461,1678).
  *   tb: nt signed distances b (OFF only), tx0/tn0: 3*nt nearest surface point
  *       and its normal (OFF only; tx0 used by DL only).
+ *   HOR_P_OFF takes the SL_OFF inputs; its result has one value per target.
  *   ctr: root-box centre for HOR_TREE (KITC.cpp:45-60), ignored otherwise.
  * ------------------------------------------------------------------------ */
 typedef struct hor_plan hor_plan;
@@ -114,14 +118,15 @@ int hor_plan_create(hor_ctx* ctx, int kind, int method, int precision,
 int hor_plan_destroy(hor_plan* plan);
 
 /* Evaluates the plan's operator.
- *   dens: 3*ns source density (f for SL, g for DL), host.
- *   d0:   3*nt density at each target's nearest point (f0 = nrst_f for SL_OFF,
- *         g0 = nrst_g for DL_OFF), host; ignored for *_ON.
- *   out:  3*nt result, host. */
+ *   dens: 3*ns source density (f for SL and P, g for DL), host.
+ *   d0:   3*nt density at each target's nearest point (f0 = nrst_f for SL_OFF
+ *         and P_OFF, g0 = nrst_g for DL_OFF), host; ignored for *_ON.
+ *   out:  3*nt result (nt for P_OFF), host. */
 int hor_plan_eval(hor_plan* plan, const double* dens, const double* d0, double* out);
 
 /* Device-resident variant: all three pointers are device pointers
- * (hor_dev_alloc); dens_dev 3*ns, d0_dev 3*nt or NULL, out_dev 3*nt. */
+ * (hor_dev_alloc); dens_dev 3*ns, d0_dev 3*nt or NULL, out_dev 3*nt (nt for
+ * P_OFF). */
 int hor_plan_eval_dev(hor_plan* plan, const double* dens_dev, const double* d0_dev,
                       double* out_dev);
 
@@ -172,6 +177,19 @@ int hor_dl_offsurf_tree(hor_ctx* ctx, const double* ctr, const hor_tree_params* 
                         const double* sa, const double* sg, int nt,
                         const double* tx, const double* tb, const double* tx0,
                         const double* tn0, const double* tg0, double* out);
+
+/* stokes_pressure_7ord (extend_to_grid/stokes.h:24-28) at nt targets: sf the
+ * surface force, tb/tn0/tf0 the signed distance, nearest normal and the
+ * force there; out: nt pressures. */
+int hor_pressure_7ord(hor_ctx* ctx, double DEL, int ns, const double* sx,
+                      const double* sn, const double* sa, const double* sf, int nt,
+                      const double* tx, const double* tb, const double* tn0,
+                      const double* tf0, double* out);
+/* stokes_pressure_away (extend_to_grid/stokes.h:20-22): the plain sum
+ * 1/(4 pi) sum A f.dx / r^3 at nt targets away from the surface. */
+int hor_pressure_away(hor_ctx* ctx, int ns, const double* sx, const double* sn,
+                      const double* sa, const double* sf, int nt, const double* tx,
+                      double* out);
 
 /* ------------------------------------------------------------------------
  * Two-surface second-kind solve (highreg_stokes_2surf/main.cpp:140-677).

@@ -348,6 +348,86 @@ void orc_dl_on(double DEL, int ns, const double* sx, const double* sn,
 }
 
 /* ======================================================================
+ * Regularized pressure (highreg_stokes_extend_to_grid/stokes.cpp:99-203)
+ * ====================================================================== */
+
+/* Lapl_DL_7ord (extend_to_grid/utilities.cpp:190-211). */
+void orc_lapl_dl_kernel(double r, double d, const double c[3], double* s2) {
+  const double rp = rootpi();
+  if (r < 1e-14) {
+    *s2 = 256.0 / 15.0 / rp / (d * d * d);
+  } else if (r / d > 8.0) {
+    *s2 = 1.0 / (r * r * r);
+  } else {
+    double rd = r / d, rd2 = rd * rd, rd4 = rd2 * rd2;
+    double ef = erf(rd);
+    double rex = rd * exp(-rd2);
+    double p2 = -2.0 / rp + 2.0 * (c[0] + 2.0 * (c[1] + c[2])) * rd2
+                - 4.0 * (c[1] + 5.0 * c[2]) * rd4 + 8.0 * c[2] * rd4 * rd2;
+    *s2 = (ef + p2 * rex) / (r * r * r);
+  }
+}
+
+/* stokes_pressure_7ord (stokes.cpp:134-203): p = -1/(4 pi) sum_j
+ * [-(f.n - f0.n0)(n.dx) - (n x f - n0 x f0).(n x dx)] H2 A_j, minus f0.n0
+ * for b < -1e-12.  Every pair goes through Lapl_DL_7ord (no per-target far
+ * class). */
+void orc_pressure_off(double DEL, int ns, const double* sx, const double* sn,
+                      const double* sa, const double* sf, int nt, const double* tx,
+                      const double* tb, const double* tn0, const double* tf0, double* out) {
+  const double s4 = 1.0 / (4.0 * ORC_PI);
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int i = 0; i < nt; i++) {
+    const double* f0 = tf0 + 3 * i;
+    const double* n0 = tn0 + 3 * i;
+    double b = tb[i];
+    double c[3];
+    orc_reg_coeff(b / DEL, c);
+    double f0n0 = f0[0] * n0[0] + f0[1] * n0[1] + f0[2] * n0[2];
+    double m0[3] = {n0[1] * f0[2] - n0[2] * f0[1], n0[2] * f0[0] - n0[0] * f0[2],
+                    n0[0] * f0[1] - n0[1] * f0[0]};
+    double p = 0.0;
+    for (int j = 0; j < ns; j++) {
+      double dx[3] = {tx[3 * i] - sx[3 * j], tx[3 * i + 1] - sx[3 * j + 1], tx[3 * i + 2] - sx[3 * j + 2]};
+      double r = sqrt(dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2]);
+      double H2;
+      orc_lapl_dl_kernel(r, DEL, c, &H2);
+      const double* n = sn + 3 * j;
+      const double* f = sf + 3 * j;
+      double fn = f[0] * n[0] + f[1] * n[1] + f[2] * n[2];
+      double nf[3] = {n[1] * f[2] - n[2] * f[1] - m0[0], n[2] * f[0] - n[0] * f[2] - m0[1],
+                      n[0] * f[1] - n[1] * f[0] - m0[2]};
+      double ndx = n[0] * dx[0] + n[1] * dx[1] + n[2] * dx[2];
+      double ndx3[3] = {n[1] * dx[2] - n[2] * dx[1], n[2] * dx[0] - n[0] * dx[2], n[0] * dx[1] - n[1] * dx[0]};
+      double normal = -(fn - f0n0) * ndx;
+      double tangential = -(nf[0] * ndx3[0] + nf[1] * ndx3[1] + nf[2] * ndx3[2]);
+      p += (normal + tangential) * H2 * sa[j];
+    }
+    p *= -s4;
+    if (b < -1.e-12) p -= f0n0;
+    out[i] = p;
+  }
+}
+
+/* stokes_pressure_away (stokes.cpp:102-127): p = 1/(4 pi) sum_j A_j f.dx / r^3. */
+void orc_pressure_away(int ns, const double* sx, const double* sa, const double* sf, int nt,
+                       const double* tx, double* out) {
+  const double s4 = 1.0 / (4.0 * ORC_PI);
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int i = 0; i < nt; i++) {
+    double p = 0.0;
+    for (int j = 0; j < ns; j++) {
+      double dx = tx[3 * i] - sx[3 * j], dy = tx[3 * i + 1] - sx[3 * j + 1], dz = tx[3 * i + 2] - sx[3 * j + 2];
+      double r2 = dx * dx + dy * dy + dz * dz;
+      double H2 = 1.0 / (r2 * sqrt(r2));
+      H2 *= sf[3 * j] * dx + sf[3 * j + 1] * dy + sf[3 * j + 2] * dz;
+      p += H2 * sa[j];
+    }
+    out[i] = p * s4;
+  }
+}
+
+/* ======================================================================
  * KITC treecode
  * ====================================================================== */
 

@@ -20,6 +20,9 @@
  *                          convergence to the direct sum as P grows, and the
  *                          barycentric identity on polynomial densities.
  *   stencil/interp       : reference build; exactness on degree-5 polynomials.
+ *   pressure             : reference build (stokes_pressure_7ord/_away);
+ *                          Lapl_DL_7ord = the s2 of Stokes_SL_7ord; the
+ *                          translating spheroid's exact pressure.
  *   matvec, RHS          : reference build (solve_IE_1step, eval_SL).
  *   GMRES                : reference build full solve at h=1/16; dense
  *                          numpy solve on a small system.
@@ -53,6 +56,15 @@ void orc_dl_off(double DEL, int ns, const double* sx, const double* sn,
 void orc_dl_on(double DEL, int ns, const double* sx, const double* sn,
                const double* sa, const double* sg, int nt, const double* tx,
                double* out);
+
+/* ---- regularized pressure (highreg_stokes_extend_to_grid/stokes.cpp:99-203) ----
+ * out: nt scalars.  tf0/tn0: force and normal at each target's nearest point. */
+void orc_lapl_dl_kernel(double r, double d, const double c[3], double* s2);
+void orc_pressure_off(double DEL, int ns, const double* sx, const double* sn,
+                      const double* sa, const double* sf, int nt, const double* tx,
+                      const double* tb, const double* tn0, const double* tf0, double* out);
+void orc_pressure_away(int ns, const double* sx, const double* sa, const double* sf, int nt,
+                       const double* tx, double* out);
 
 /* ---- KITC treecode (KITC.h, KITC.cpp) ----
  * ctr: centre of the root box [ctr-1-1e-8, ctr+1+1e-8]^3 (KITC.cpp:30-69).

@@ -70,7 +70,8 @@ class Oracle:
         for name in ("orc_sl_off", "orc_sl_on", "orc_dl_off", "orc_dl_on",
                      "orc_sl_on_tree", "orc_sl_off_tree", "orc_dl_on_tree",
                      "orc_dl_off_tree", "orc_stencil", "orc_interp_g0",
-                     "orc_matvec", "orc_eval_sl", "orc_reg_coeff", "orc_In_024"):
+                     "orc_matvec", "orc_eval_sl", "orc_reg_coeff", "orc_In_024",
+                     "orc_pressure_off", "orc_pressure_away", "orc_lapl_dl_kernel"):
             getattr(L, name).restype = None
         L.orc_gmres.restype = _I
         L.orc_tree_structure.restype = _I
@@ -88,7 +89,7 @@ class Oracle:
         return out
 
     def kernel(self, kind, r, d, c=(0.0, 0.0, 0.0)):
-        """kind: 'sl', 'sl_on', 'dl', 'dl_on' -> array (len(r), 2)."""
+        """kind: 'sl', 'sl_on', 'dl', 'dl_on', 'lapl_dl' -> array (len(r), 2)."""
         L = self.lib
         cc = _f64(np.asarray(c))
         out = np.zeros((len(r), 2))
@@ -100,6 +101,9 @@ class Oracle:
                 L.orc_sl_kernel_on(_D(rv), _D(d), C.byref(a), C.byref(b))
             elif kind == "dl":
                 L.orc_dl_kernel(_D(rv), _D(d), _p(cc), C.byref(a), C.byref(b))
+            elif kind == "lapl_dl":
+                a.value = 0.0
+                L.orc_lapl_dl_kernel(_D(rv), _D(d), _p(cc), C.byref(b))
             else:
                 a.value = 0.0
                 L.orc_dl_kernel_on(_D(rv), _D(d), C.byref(b))
@@ -133,6 +137,22 @@ class Oracle:
         out = np.zeros(3 * nt)
         self.lib.orc_dl_on(_D(DEL), src["n"], _p(src["x"]), _p(src["nrml"]), _p(src["area"]),
                            _p(src["g"]), nt, _p(src["x"]), _p(out))
+        return out
+
+    # ---- pressure
+    def pressure_off(self, DEL, src, tgt):
+        """stokes_pressure_7ord at each target; src['f'] the force, tgt['f0'] at x0."""
+        out = np.zeros(tgt["n"])
+        self.lib.orc_pressure_off(_D(DEL), src["n"], _p(src["x"]), _p(src["nrml"]), _p(src["area"]),
+                                  _p(src["f"]), tgt["n"], _p(tgt["x"]), _p(tgt["b"]), _p(tgt["n0"]),
+                                  _p(tgt["f0"]), _p(out))
+        return out
+
+    def pressure_away(self, src, tx):
+        tx = _f64(tx)
+        out = np.zeros(len(tx) // 3)
+        self.lib.orc_pressure_away(src["n"], _p(src["x"]), _p(src["area"]), _p(src["f"]), len(out), _p(tx),
+                                   _p(out))
         return out
 
     # ---- treecode
@@ -448,6 +468,26 @@ class RefGrid:
         self.lib.refg_sl_7ord(_D(DEL), src["n"], _p(src["x"]), _p(src["nrml"]), _p(src["area"]),
                               _p(src["f"]), nt, _p(tx), _p(tb), _p(f0n0), _p(out))
         return out
+
+    def pressure_7ord(self, DEL, src, tgt):
+        out = np.zeros(tgt["n"])
+        self.lib.refg_pressure_7ord(_D(DEL), src["n"], _p(src["x"]), _p(src["nrml"]), _p(src["area"]),
+                                    _p(src["f"]), tgt["n"], _p(_f64(tgt["x"])), _p(_f64(tgt["b"])),
+                                    _p(_f64(tgt["n0"])), _p(_f64(tgt["f0"])), _p(out))
+        return out
+
+    def pressure_away(self, src, tx):
+        tx = _f64(tx)
+        out = np.zeros(len(tx) // 3)
+        self.lib.refg_pressure_away(src["n"], _p(src["x"]), _p(src["area"]), _p(src["f"]), len(out), _p(tx),
+                                    _p(out))
+        return out
+
+    def p_exact(self, x):
+        x = _f64(x)
+        p = np.zeros(len(x) // 3)
+        self.lib.refg_p_exact(len(p), _p(x), _p(p))
+        return p
 
     def density(self, x):
         x = _f64(x)

@@ -117,6 +117,45 @@ void refg_sl_away(int ns, const double* sx, const double* sa, const double* sf,
   }
 }
 
+// stokes_pressure_7ord (stokes.cpp:134-203) at each target: f0[3n] the force
+// at the nearest point, n0[3n] its normal.
+void refg_pressure_7ord(double DEL, int ns, const double* sx, const double* sn,
+                        const double* sa, const double* sf, int nt, const double* tx,
+                        const double* tb, const double* tn0, const double* tf0, double* out) {
+  vector<Surf_point> s(ns);
+  for (int i = 0; i < ns; i++) {
+    for (int k = 0; k < 3; k++) {
+      s[i].x[k] = sx[3 * i + k];
+      s[i].Nrml[k] = sn[3 * i + k];
+    }
+    s[i].Area = sa[i];
+  }
+  vector<double> f(sf, sf + 3 * ns);
+#pragma omp parallel for
+  for (int t = 0; t < nt; t++) {
+    vector<double> f0(tf0 + 3 * t, tf0 + 3 * t + 3), n0(tn0 + 3 * t, tn0 + 3 * t + 3);
+    out[t] = stokes_pressure_7ord(DEL, ns, s, f, f0, n0, tx[3 * t], tx[3 * t + 1], tx[3 * t + 2], tb[t]);
+  }
+}
+
+// stokes_pressure_away (stokes.cpp:102-127).
+void refg_pressure_away(int ns, const double* sx, const double* sa, const double* sf,
+                        int nt, const double* tx, double* out) {
+  vector<Surf_point> s(ns);
+  for (int i = 0; i < ns; i++) {
+    for (int k = 0; k < 3; k++) s[i].x[k] = sx[3 * i + k];
+    s[i].Area = sa[i];
+  }
+  vector<double> f(sf, sf + 3 * ns);
+#pragma omp parallel for
+  for (int t = 0; t < nt; t++) out[t] = stokes_pressure_away(ns, s, f, tx[3 * t], tx[3 * t + 1], tx[3 * t + 2]);
+}
+
+// p_exact (utilities.cpp:294-321).
+void refg_p_exact(int n, const double* x, double* p) {
+  for (int i = 0; i < n; i++) p[i] = p_exact(x[3 * i], x[3 * i + 1], x[3 * i + 2]);
+}
+
 // stokeslet_density (utilities.cpp:267-285) and SL_exact (360-411).
 void refg_density(int n, const double* x, double* f) {
   for (int i = 0; i < n; i++) {

@@ -6,7 +6,8 @@ Inputs come from the reference's own geometry routines (Generate_Surface,
 Compute_nearest_data, set_surface_force, find_nrst_surfc) and seeded
 densities; outputs are the reference functions' results.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py            # every fixture
+    python tests/golden/make_golden.py pressure   # grid_pressure_h16.npz only
 """
 import os
 import sys
@@ -69,7 +70,33 @@ def two_surface(R, N, seed):
     return out
 
 
+def grid_pressure(G):
+    """stokes_pressure_7ord at the band nodes of the grid program (h = 1/16,
+    every band node, inside ones too), with f the translating spheroid's
+    traction and f0 = f(x0); stokes_pressure_away at the nodes of two box
+    faces; p_exact at both."""
+    N = 48
+    h = 3.0 / N
+    DEL = delta(h)
+    s = G.generate_surface(N, h)
+    s["f"] = G.density(s["x"])
+    bt = G.band_targets(N, h, 4.0, 2.0, s)
+    tx = (bt["ijk"] * h).astype(np.float64).ravel()
+    tgt = dict(n=bt["n"], x=tx, b=bt["b"], n0=bt["n0"], f0=G.density(bt["x0"]))
+    p_band = G.pressure_7ord(DEL, s, tgt)
+    g = np.arange(N + 1) * h
+    face = np.array([(x, y, z) for x in (0.0, N * h) for y in g for z in g])
+    p_face = G.pressure_away(s, face.ravel())
+    np.savez_compressed(os.path.join(HERE, "grid_pressure_h16.npz"), N=N, h=h, DEL=DEL, ijk=bt["ijk"],
+                        b=bt["b"], n0=bt["n0"], f0=tgt["f0"], p_band=p_band, p_band_exact=G.p_exact(tx),
+                        face=face.ravel(), p_face=p_face, p_face_exact=G.p_exact(face.ravel()))
+
+
 def main():
+    if sys.argv[1:] == ["pressure"]:
+        grid_pressure(RefGrid())
+        print("grid_pressure_h16.npz written to", HERE)
+        return
     R = Ref2()
     K = R.constants()
     # scalar kernels and coefficients
@@ -97,6 +124,7 @@ def main():
                         s_x=s["x"], s_nrml=s["nrml"], s_area=s["area"], ijk=bt["ijk"], b=bt["b"],
                         x0=bt["x0"], n0=bt["n0"], band2=bt["band2"], f=G.density(s["x"]),
                         exact=G.sl_exact(tx), outside=outside)
+    grid_pressure(G)
     print("golden fixtures written to", HERE)
 
 
