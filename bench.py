@@ -355,6 +355,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-gmres", action="store_true", help="skip the two-sphere GMRES workload")
+    ap.add_argument("--no-grid", action="store_true", help="skip the grid-program leg")
     ap.add_argument("--no-h256", action="store_true", help="skip the h=1/256 two-sphere GMRES workload")
     ap.add_argument("--cpu-h256", action="store_true",
                     help="also time two CPU treecode matvecs and the RHS at h=1/256 (minutes of CPU time)")
@@ -475,6 +476,18 @@ def main():
     if not args.no_cpu:
         pressure["cpu_baseline"] = pressure_cpu_baseline(s, t, DEL)
 
+    # the grid program end to end: boundary-integral pressure and velocity on
+    # the GPU, extension to the whole grid on the host
+    grid = None
+    if not args.no_grid:
+        from hor import grid as HG
+        log("grid program (translating spheroid, h=1/64)")
+        t0 = time.perf_counter()
+        r = HG.grid_program(ctx, N=192)
+        grid = {"workload": "grid program: spheroid (1,0.5,0.5) translating in [0,3]^3, h=1/64 (193^3 nodes)",
+                "sources": r["sources"], "band_nodes": r["band_nodes"], "seconds": time.perf_counter() - t0,
+                "times_s": r["times"], "errors": r["errors"]}
+
     cpu = None if args.no_cpu else cpu_baseline(s, t, DEL)
     gm = None
     if not args.no_gmres:
@@ -511,6 +524,7 @@ def main():
                   "rel_err_vs_fp64_sl": err_sl, "rel_err_vs_fp64_dl": err_dl},
         "accuracy": {"sl_max_err_inside_vs_exact_U": err_inside},
         "pressure": pressure,
+        "grid_program": grid,
         "gmres": gm,
     }
     for p in dev.values():

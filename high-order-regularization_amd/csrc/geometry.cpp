@@ -374,6 +374,27 @@ void spheroid_exact(const hor_body& B, int n, const double* xin, double* uout) {
   }
 }
 
+// p_exact (extend_to_grid/utilities.cpp:294-321): pressure of the prolate
+// spheroid translating with U = (1,0,0) along its axis, p = -2 alpha1
+// (1/R1 - 1/R2) (Chwang & Wu, eq. 29a), in the body frame.
+void spheroid_exact_pressure(const hor_body& B, int n, const double* xin, double* pout) {
+  double a_sq = B.abc[0] * B.abc[0], b_sq = B.abc[1] * B.abc[1];
+  double c = std::sqrt(a_sq - b_sq);
+  double e = c / B.abc[0];
+  double Le = std::log((1.0 + e) / (1.0 - e));
+  double a1 = e * e / ((1.0 + e * e) * Le - 2.0 * e);
+  for (int p = 0; p < n; p++) {
+    double u[3];
+    to_body(B, xin + 3 * p, true, u);
+    double x = u[0], y = u[1], z = u[2];
+    double r_sq = y * y + z * z;
+    double R1 = std::sqrt((x + c) * (x + c) + r_sq);
+    double R2 = std::sqrt((x - c) * (x - c) + r_sq);
+    double dB1dx = 1.0 / R1 - 1.0 / R2;
+    pout[p] = -2.0 * a1 * dB1dx;
+  }
+}
+
 }  // namespace hor
 
 // ---------------------------------------------------------------------------
@@ -449,6 +470,22 @@ int hor_band_targets(const hor_body* body, const double* origin, double h, int n
 int hor_traction_translating(const hor_body* body, const double* U, int n, const double* x, double* f) {
   if (!body || !U || !x || !f || n < 0) return hor::set_error(HOR_E_ARG, "hor_traction_translating: bad argument");
   hor::traction_translating(*body, U, n, x, f);
+  return HOR_OK;
+}
+
+int hor_spheroid_exact_pressure(const hor_body* body, int n, const double* x, double* p) {
+  if (!body || !x || !p || n < 0) return hor::set_error(HOR_E_ARG, "hor_spheroid_exact_pressure: bad argument");
+  if (!(body->abc[0] > body->abc[1] && body->abc[1] == body->abc[2]))
+    return hor::set_error(HOR_E_ARG, "hor_spheroid_exact_pressure: needs a prolate spheroid a > b = c");
+  hor::spheroid_exact_pressure(*body, n, x, p);
+  return HOR_OK;
+}
+
+// phi (utilities.cpp:58-86) at n points.
+int hor_level_set(const hor_body* body, int n, const double* x, double* phi_out) {
+  if (!body || !x || !phi_out || n < 0) return hor::set_error(HOR_E_ARG, "hor_level_set: bad argument");
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; i++) phi_out[i] = hor::geom::phi(*body, x + 3 * i, true);
   return HOR_OK;
 }
 

@@ -25,14 +25,15 @@ int band_targets(const hor_body& B, const double* origin, double h, int ngrid, d
                  std::vector<double>& b, std::vector<double>& X0, std::vector<double>& N0);
 void traction_translating(const hor_body& B, const double U[3], int n, const double* x, double* f);
 void spheroid_exact(const hor_body& B, int n, const double* x, double* u);
+void spheroid_exact_pressure(const hor_body& B, int n, const double* x, double* p);
 
-// ---- regularization coefficients (coeffs.cpp): evaluate_In_024 +
+// ---- regularization coefficients (common.cpp): evaluate_In_024 +
 // get_reg_coeff_7ord (highreg_stokes_2surf/utilities.cpp:163-175, 327-337),
 // evaluated on the host with the C library's erfc/exp so that the GPU uses
 // exactly the coefficients the reference computes.
 void reg_coeff_7ord(double lam, double c[3]);
 
-// ---- spatial ordering (order.cpp)
+// ---- spatial ordering (common.cpp)
 // Morton key of p inside the box [lo, lo+ext]^3 (21 bits per axis).
 uint64_t morton_key(const double p[3], const double lo[3], double ext);
 // Stable order of n points by (cls[i], Morton key); cls may be null.

@@ -15,6 +15,7 @@ _DP = C.POINTER(C.c_double)
 
 HOR_OK = 0
 SL_ON, SL_OFF, DL_ON, DL_OFF, P_OFF = 0, 1, 2, 3, 4
+ALL_NEAR, ON_AT = 0x10, 0x20   # kind modifier bits (hor.h)
 DIRECT, TREE = 0, 1
 FP64, MIXED = 0, 1
 
@@ -56,6 +57,10 @@ _SIGS = {
     "hor_dl_onsurf_tree": (_I, [_P, _P, C.POINTER(TreeParams), _D, _I, _P, _P, _P, _P, _I, _P, _P]),
     "hor_dl_offsurf_tree": (_I, [_P, _P, C.POINTER(TreeParams), _D, _I, _P, _P, _P, _P, _I, _P,
                                  _P, _P, _P, _P, _P]),
+    "hor_spheroid_exact_pressure": (_I, [C.POINTER(Body), _I, _P, _P]),
+    "hor_level_set": (_I, [C.POINTER(Body), _I, _P, _P]),
+    "hor_dst_poisson3d": (_I, [_I, _P]),
+    "hor_grid_solve_poisson": (_I, [_I, _D, _P, _P, _P, _P, _I]),
     "hor_pressure_7ord": (_I, [_P, _D, _I, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P]),
     "hor_pressure_away": (_I, [_P, _I, _P, _P, _P, _P, _I, _P, _P]),
     "hor_solver_create": (_I, [_P, _D, _D, _I, _I, C.POINTER(TreeParams), C.POINTER(TreeParams),

@@ -398,6 +398,32 @@ int hor_traction_translating(const hor_body* body, const double* U, int n,
                              const double* x, double* f);
 int hor_spheroid_exact_velocity(const hor_body* body, int n, const double* x,
                                 double* u);
+/* p_exact (extend_to_grid/utilities.cpp:294-321): pressure of that spheroid. */
+int hor_spheroid_exact_pressure(const hor_body* body, int n, const double* x,
+                                double* p);
+/* The body's level set phi (utilities.cpp:58-86) at n points (host). */
+int hor_level_set(const hor_body* body, int n, const double* x, double* phi);
+
+/* ------------------------------------------------------------------------
+ * Grid extension (host, OpenMP; highreg_stokes_extend_to_grid/main.cpp).
+ * Grid nodes (i,j,k), 0 <= i,j,k <= N, at (i,j,k) L/N, stored at
+ * (i (N+1) + j) (N+1) + k.
+ * ------------------------------------------------------------------------ */
+/* fftw_DST_Poisson3d_FD (main.cpp:1423-1491): f[n^3] = h^2 times the right-hand
+ * side at the interior nodes (row-major) in; the solution of the 4th-order
+ * discrete Poisson problem with zero Dirichlet data out (DST-I through the
+ * library's own FFT; no FFTW). */
+int hor_dst_poisson3d(int n, double* f);
+
+/* solve_Poisson (main.cpp:969-1044): extends u[(N+1)^3], given on the box
+ * faces and at the band nodes, to the whole grid in place.
+ *   band[(N+1)^3]: 0 outside BAND1, 1 in BAND1 only, 2 in BAND2 (|b| <= BAND2 h);
+ *   outside[(N+1)^3]: 1 where phi > 0;
+ *   variable: 0 for the pressure (no forcing), 1..3 for a velocity component
+ *             (forcing dp/dx_variable from p[(N+1)^3] outside the body and BAND2). */
+int hor_grid_solve_poisson(int N, double L, const signed char* band,
+                           const unsigned char* outside, const double* p,
+                           double* u, int variable);
 
 /* k nearest neighbours of each of n points (host): nbr[i*k + a], sorted by
  * (distance, index), the point itself excluded; the graph of the learned

@@ -1204,3 +1204,175 @@ int orc_gmres(const orc_problem* pb, int n, const double* b, double* x, int im,
   free(V); free(H); free(g); free(y); free(c); free(s); free(Ax); free(xbase); free(w);
   return iter;
 }
+
+/* ======================================================================
+ * Grid extension (highreg_stokes_extend_to_grid/main.cpp:969-1830)
+ * ====================================================================== */
+
+#define GAT(N1, i, j, k) ((((size_t)(i)) * (N1) + (j)) * (N1) + (k))
+#define GIN(N, i, j, k) ((((size_t)(i) - 1) * ((N) - 1) + (j) - 1) * ((N) - 1) + (k) - 1)
+
+/* DST-I along one axis of an n^3 array with the dense sine matrix S. */
+static void orc_dst_axis(int n, const double* S, double* f, int axis) {
+  const size_t n2 = (size_t)n * n;
+  const size_t stride = axis == 2 ? 1 : axis == 1 ? (size_t)n : n2;
+#pragma omp parallel
+  {
+    double* x = (double*)malloc(sizeof(double) * n);
+    double* y = (double*)malloc(sizeof(double) * n);
+#pragma omp for schedule(static)
+    for (long l = 0; l < (long)n2; l++) {
+      /* line base: the two other indices of l */
+      int a = (int)(l / n), b = (int)(l % n);
+      size_t base = axis == 2 ? ((size_t)a * n + b) * n : axis == 1 ? (size_t)a * n2 + b : (size_t)a * n + b;
+      for (int t = 0; t < n; t++) x[t] = f[base + t * stride];
+      for (int k = 0; k < n; k++) {
+        double acc = 0.0;
+        for (int j = 0; j < n; j++) acc += x[j] * S[(size_t)j * n + k];
+        y[k] = acc;
+      }
+      for (int t = 0; t < n; t++) f[base + t * stride] = y[t];
+    }
+    free(x);
+    free(y);
+  }
+}
+
+/* fftw_DST_Poisson3d_FD (main.cpp:1423-1491): forward DST-I (FFTW RODFT00,
+ * Y_k = 2 sum x_j sin(pi (j+1)(k+1)/(n+1))) on all axes, division by the
+ * eigenvalue factor, the same transform again, scale 1/(2(n+1))^3. */
+void orc_dst_poisson3d(int n, double* f) {
+  double* S = (double*)malloc(sizeof(double) * n * n);
+  for (int j = 0; j < n; j++)
+    for (int k = 0; k < n; k++) S[(size_t)j * n + k] = 2.0 * sin(ORC_PI * (j + 1.0) * (k + 1.0) / (n + 1.0));
+  for (int ax = 0; ax < 3; ax++) orc_dst_axis(n, S, f, ax);
+  double pi_over_n = ORC_PI / (n + 1);
+  for (int i = 0; i < n; i++) {
+    double c1 = cos((i + 1) * pi_over_n);
+    double s1 = sin((i + 1) * pi_over_n / 2.0);
+    s1 *= -2.0 * s1;
+    for (int j = 0; j < n; j++) {
+      double c2 = cos((j + 1) * pi_over_n);
+      double s2 = sin((j + 1) * pi_over_n / 2.0);
+      s2 *= -2.0 * s2;
+      for (int k = 0; k < n; k++) {
+        double c3 = cos((k + 1) * pi_over_n);
+        double s3 = sin((k + 1) * pi_over_n / 2.0);
+        s3 *= -2.0 * s3;
+        size_t ind = ((size_t)i * n + j) * n + k;
+        double lap7 = s1 + s2 + s3;
+        double sum8 = c1 * c2 * c3;
+        double factor = 2.0 / 3.0 * (2.0 * lap7 + sum8 - 1.0);
+        if (fabs(factor) > 1.e-14) f[ind] /= factor;
+        else f[ind] = 0.0;
+      }
+    }
+  }
+  for (int ax = 0; ax < 3; ax++) orc_dst_axis(n, S, f, ax);
+  double scale = 1. / pow(2 * (n + 1), 3);
+  for (size_t q = 0; q < (size_t)n * n * n; q++) f[q] *= scale;
+  free(S);
+}
+
+/* extend_into_cube (main.cpp:1494-1563). */
+void orc_extend_into_cube(int N, double L, double h, const double* g, double* G) {
+  const size_t N1 = (size_t)N + 1;
+  for (int i = 0; i <= N; i++) {
+    double x = i * h / L;
+    for (int j = 0; j <= N; j++) {
+      double y = j * h / L;
+      for (int k = 0; k <= N; k++) {
+        double z = k * h / L;
+        double F, E1, E2, E3, C;
+        F = x * g[GAT(N1, N, j, k)] + (1.0 - x) * g[GAT(N1, 0, j, k)];
+        F += y * g[GAT(N1, i, N, k)] + (1.0 - y) * g[GAT(N1, i, 0, k)];
+        F += z * g[GAT(N1, i, j, N)] + (1.0 - z) * g[GAT(N1, i, j, 0)];
+        E1 = (1.0 - y) * (1.0 - z) * g[GAT(N1, i, 0, 0)] + (1.0 - y) * z * g[GAT(N1, i, 0, N)]
+           + y * (1.0 - z) * g[GAT(N1, i, N, 0)] + y * z * g[GAT(N1, i, N, N)];
+        E2 = (1.0 - x) * (1.0 - z) * g[GAT(N1, 0, j, 0)] + (1.0 - x) * z * g[GAT(N1, 0, j, N)]
+           + x * (1.0 - z) * g[GAT(N1, N, j, 0)] + x * z * g[GAT(N1, N, j, N)];
+        E3 = (1.0 - x) * (1.0 - y) * g[GAT(N1, 0, 0, k)] + (1.0 - x) * y * g[GAT(N1, 0, N, k)]
+           + x * (1.0 - y) * g[GAT(N1, N, 0, k)] + x * y * g[GAT(N1, N, N, k)];
+        C = (1.0 - x) * (1.0 - y) * (1.0 - z) * g[GAT(N1, 0, 0, 0)] + (1.0 - x) * (1.0 - y) * z * g[GAT(N1, 0, 0, N)]
+          + (1.0 - x) * y * (1.0 - z) * g[GAT(N1, 0, N, 0)] + (1.0 - x) * y * z * g[GAT(N1, 0, N, N)]
+          + x * (1.0 - y) * (1.0 - z) * g[GAT(N1, N, 0, 0)] + x * (1.0 - y) * z * g[GAT(N1, N, 0, N)]
+          + x * y * (1.0 - z) * g[GAT(N1, N, N, 0)] + x * y * z * g[GAT(N1, N, N, N)];
+        G[GAT(N1, i, j, k)] = F - E1 - E2 - E3 + C;
+      }
+    }
+  }
+}
+
+static double orc_lap4(int N, const double* G, int i, int j, int k) {
+  const size_t N1 = (size_t)N + 1;
+  double lap7 = G[GAT(N1, i + 1, j, k)] + G[GAT(N1, i - 1, j, k)] + G[GAT(N1, i, j + 1, k)] + G[GAT(N1, i, j - 1, k)]
+              + G[GAT(N1, i, j, k + 1)] + G[GAT(N1, i, j, k - 1)] - 6.0 * G[GAT(N1, i, j, k)];
+  double temp1 = G[GAT(N1, i + 1, j + 1, k + 1)] + G[GAT(N1, i + 1, j - 1, k + 1)]
+               + G[GAT(N1, i - 1, j + 1, k + 1)] + G[GAT(N1, i - 1, j - 1, k + 1)] - 4.0 * G[GAT(N1, i, j, k)];
+  double temp2 = G[GAT(N1, i + 1, j + 1, k - 1)] + G[GAT(N1, i + 1, j - 1, k - 1)]
+               + G[GAT(N1, i - 1, j + 1, k - 1)] + G[GAT(N1, i - 1, j - 1, k - 1)] - 4.0 * G[GAT(N1, i, j, k)];
+  return 2.0 / 3.0 * (lap7 + (temp1 + temp2) / 8.0);
+}
+
+/* lapl_4ord_inCube (main.cpp:1566-1595). */
+void orc_lapl_in_cube(int N, const double* G, double* rhs) {
+  for (int i = 1; i < N; i++)
+    for (int j = 1; j < N; j++)
+      for (int k = 1; k < N; k++) rhs[GIN(N, i, j, k)] = -orc_lap4(N, G, i, j, k);
+}
+
+/* lapl_4ord_nearSurfc (main.cpp:1598-1637): band == 2 marks BAND2 nodes. */
+void orc_lapl_near_surface(int N, const signed char* band, const double* u, double* rhs) {
+  const size_t N1 = (size_t)N + 1;
+  for (int i = 1; i < N; i++)
+    for (int j = 1; j < N; j++)
+      for (int k = 1; k < N; k++)
+        if (band[GAT(N1, i, j, k)] == 2) rhs[GIN(N, i, j, k)] += orc_lap4(N, u, i, j, k);
+}
+
+/* dp_dx_4ord / dp_dy_4ord / dp_dz_4ord (main.cpp:1641-1830). */
+void orc_dp_4ord(int N, double h, int axis, const signed char* band, const unsigned char* outside,
+                 const double* p, double* rhs) {
+  const size_t N1 = (size_t)N + 1;
+  for (int i = 1; i < N; i++)
+    for (int j = 1; j < N; j++)
+      for (int k = 1; k < N; k++) {
+        size_t q = GAT(N1, i, j, k);
+        int out_band2 = outside[q] && band[q] != 2;
+        if (!out_band2) continue;
+        int di = axis == 0, dj = axis == 1, dk = axis == 2;
+        int c = axis == 0 ? i : axis == 1 ? j : k;
+#define PV(o) p[GAT(N1, i + (o) * di, j + (o) * dj, k + (o) * dk)]
+        double d;
+        if (c == 0 || c == 1)
+          d = -(25.0 / 12.0) * PV(0) + 4.0 * PV(1) - 3.0 * PV(2) + (4.0 / 3.0) * PV(3) - 0.25 * PV(4);
+        else if (c == N || c == (N - 1))
+          d = (25.0 / 12.0) * PV(0) - 4.0 * PV(-1) + 3.0 * PV(-2) - (4.0 / 3.0) * PV(-3) + 0.25 * PV(-4);
+        else {
+          d = -PV(2) + 8.0 * PV(1) - 8.0 * PV(-1) + PV(-2);
+          d /= 12.0;
+        }
+#undef PV
+        rhs[GIN(N, i, j, k)] += d * h;
+      }
+}
+
+/* solve_Poisson (main.cpp:969-1044). */
+void orc_solve_poisson(int N, double L, double h, const signed char* band, const unsigned char* outside,
+                       const double* p, double* u, int variable) {
+  const size_t N1 = (size_t)N + 1;
+  double* w = (double*)malloc(sizeof(double) * N1 * N1 * N1);
+  size_t nin = (size_t)(N - 1) * (N - 1) * (N - 1);
+  double* rhs = (double*)calloc(nin, sizeof(double));
+  orc_extend_into_cube(N, L, h, u, w);
+  orc_lapl_in_cube(N, w, rhs);
+  orc_lapl_near_surface(N, band, u, rhs);
+  if (variable >= 1 && variable <= 3) orc_dp_4ord(N, h, variable - 1, band, outside, p, rhs);
+  orc_dst_poisson3d(N - 1, rhs);
+  for (size_t q = 0; q < N1 * N1 * N1; q++) u[q] = w[q];
+  for (int i = 1; i < N; i++)
+    for (int j = 1; j < N; j++)
+      for (int k = 1; k < N; k++) u[GAT(N1, i, j, k)] += rhs[GIN(N, i, j, k)];
+  free(w);
+  free(rhs);
+}

@@ -20,6 +20,12 @@
  *                          convergence to the direct sum as P grows, and the
  *                          barycentric identity on polynomial densities.
  *   stencil/interp       : reference build; exactness on degree-5 polynomials.
+ *   grid extension       : the discrete Poisson residual (the solution satisfies
+ *                          the 4th-order stencil equations), transfinite
+ *                          exactness on face values and trilinear data,
+ *                          stencil exactness on polynomials, O(h^4)
+ *                          convergence of the harmonic extension (the
+ *                          reference's main.cpp needs FFTW: unbuildable here).
  *   pressure             : reference build (stokes_pressure_7ord/_away);
  *                          Lapl_DL_7ord = the s2 of Stokes_SL_7ord; the
  *                          translating spheroid's exact pressure.
@@ -153,6 +159,20 @@ void orc_eval_sl(const orc_problem* pb, double* sl);
 int orc_gmres(const orc_problem* pb, int n, const double* b, double* x, int m,
               double tol, int iter_max, int fix_restart, double* resid_hist,
               double* final_resid);
+
+/* ---- grid extension (highreg_stokes_extend_to_grid/main.cpp:969-1830) ----
+ * Grid (N+1)^3, node (i,j,k) at ((i N1) + j) N1 + k, N1 = N+1; interior
+ * unknowns ((i-1)(N-1) + j-1)(N-1) + k-1.  The DST-I is applied as the dense
+ * sine matrix (O(n^4)), not through an FFT. */
+void orc_dst_poisson3d(int n, double* f);
+void orc_extend_into_cube(int N, double L, double h, const double* g, double* G);
+void orc_lapl_in_cube(int N, const double* G, double* rhs);
+void orc_lapl_near_surface(int N, const signed char* band, const double* u, double* rhs);
+void orc_dp_4ord(int N, double h, int axis, const signed char* band,
+                 const unsigned char* outside, const double* p, double* rhs);
+void orc_solve_poisson(int N, double L, double h, const signed char* band,
+                       const unsigned char* outside, const double* p, double* u,
+                       int variable);
 
 #ifdef __cplusplus
 }

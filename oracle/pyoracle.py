@@ -71,7 +71,9 @@ class Oracle:
                      "orc_sl_on_tree", "orc_sl_off_tree", "orc_dl_on_tree",
                      "orc_dl_off_tree", "orc_stencil", "orc_interp_g0",
                      "orc_matvec", "orc_eval_sl", "orc_reg_coeff", "orc_In_024",
-                     "orc_pressure_off", "orc_pressure_away", "orc_lapl_dl_kernel"):
+                     "orc_pressure_off", "orc_pressure_away", "orc_lapl_dl_kernel",
+                     "orc_dst_poisson3d", "orc_extend_into_cube", "orc_lapl_in_cube",
+                     "orc_lapl_near_surface", "orc_dp_4ord", "orc_solve_poisson"):
             getattr(L, name).restype = None
         L.orc_gmres.restype = _I
         L.orc_tree_structure.restype = _I
@@ -154,6 +156,46 @@ class Oracle:
         self.lib.orc_pressure_away(src["n"], _p(src["x"]), _p(src["area"]), _p(src["f"]), len(out), _p(tx),
                                    _p(out))
         return out
+
+    # ---- grid extension (arrays are flat numpy, (N+1)^3 or (N-1)^3)
+    def dst_poisson3d(self, f):
+        f = np.array(f, dtype=np.float64)
+        n = round(len(f) ** (1 / 3))
+        self.lib.orc_dst_poisson3d(int(n), _p(f))
+        return f
+
+    def extend_into_cube(self, N, L, g):
+        G = np.zeros((N + 1) ** 3)
+        self.lib.orc_extend_into_cube(int(N), _D(L), _D(L / N), _p(_f64(g)), _p(G))
+        return G
+
+    def lapl_in_cube(self, N, G):
+        rhs = np.zeros((N - 1) ** 3)
+        self.lib.orc_lapl_in_cube(int(N), _p(_f64(G)), _p(rhs))
+        return rhs
+
+    def lapl_near_surface(self, N, band, u, rhs):
+        rhs = np.array(rhs, dtype=np.float64)
+        b = np.ascontiguousarray(band, dtype=np.int8)
+        self.lib.orc_lapl_near_surface(int(N), b.ctypes.data_as(C.c_void_p), _p(_f64(u)), _p(rhs))
+        return rhs
+
+    def dp_4ord(self, N, L, axis, band, outside, p, rhs):
+        rhs = np.array(rhs, dtype=np.float64)
+        b = np.ascontiguousarray(band, dtype=np.int8)
+        o = np.ascontiguousarray(outside, dtype=np.uint8)
+        self.lib.orc_dp_4ord(int(N), _D(L / N), int(axis), b.ctypes.data_as(C.c_void_p),
+                             o.ctypes.data_as(C.c_void_p), _p(_f64(p)), _p(rhs))
+        return rhs
+
+    def solve_poisson(self, N, L, band, outside, p, u, variable):
+        u = np.array(u, dtype=np.float64)
+        b = np.ascontiguousarray(band, dtype=np.int8)
+        o = np.ascontiguousarray(outside, dtype=np.uint8)
+        pp = _f64(p) if p is not None else np.zeros(1)
+        self.lib.orc_solve_poisson(int(N), _D(L), _D(L / N), b.ctypes.data_as(C.c_void_p),
+                                   o.ctypes.data_as(C.c_void_p), _p(pp), _p(u), int(variable))
+        return u
 
     # ---- treecode
     @staticmethod
