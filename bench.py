@@ -27,6 +27,7 @@ BAND = 0.3
 ABC = (1.0, 0.8, 0.6)
 SEED = 20261203
 PEAK_FP64 = 78.6e12          # MI355X FP64 (vector and matrix) FLOP/s
+VALU_ISSUE = 256 * 4 * 16 * 2.4e9   # VALU lane-instructions/s: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz
 # Algorithmic operation counts per pair (reference expressions, per-target
 # invariants hoisted, FMA = 2): plain SL 30, regularized off-surface SL 61,
 # plain DL 30, regularized off-surface DL 127.  The pressure pair is counted in
@@ -348,6 +349,17 @@ def profile_traffic():
     return None
 
 
+def profile_valu():
+    """Per-launch FP64/FP32 FLOPs and VALU instructions counted by rocprofv3
+    (SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64/F32, calibrated on FMA chains of
+    known length) from the committed summary."""
+    path = os.path.join(ROOT, "profiles", "direct_band_h64_valu_pmc.json")
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f).get("kernels")
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -454,6 +466,17 @@ def main():
             "avg_kernel_ms": kdl_ms, "sl_kernel_ms": ksl_ms,
             "sl_achieved_tflops": sl_flops / (ksl_ms * 1e-3) / 1e12,
             "near_pair_fraction": fnear}
+    counted = profile_valu()
+    if counted and "k_direct<DL_OFF,FP64>" in counted:
+        c = counted["k_direct<DL_OFF,FP64>"]
+        t_dl = kdl_ms * 1e-3
+        roof["counted"] = {
+            "fp64_flop_per_launch": c["fp64_flop_per_launch"],
+            "fp64_tflops": c["fp64_flop_per_launch"] / t_dl / 1e12,
+            "fp64_frac": c["fp64_flop_per_launch"] / t_dl / PEAK_FP64,
+            "valu_issue_frac": c["counters"]["SQ_INSTS_VALU"] * 64 / t_dl / VALU_ISSUE,
+            "note": "hardware-counted FP64 FLOPs (rocprofv3 VALU counters, profiles/direct_band_h64_valu_pmc.json) "
+                    "over this run's launch time; the FLOPs the kernel executes, including the 1/r refinement"}
 
     # regularized pressure (stokes_pressure_7ord) on the same band workload
     pp = hor.Plan(ctx, hor.P_OFF, s, t, DEL)
