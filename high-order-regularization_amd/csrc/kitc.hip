@@ -197,6 +197,9 @@ __global__ __launch_bounds__(kBlock) void k_leaf(LeafArgs A) {
   const int item = blockIdx.x;
   const int node = A.item_node[item];
   const int off = A.item_off[item], cnt = A.item_cnt[item];
+  // a wave without targets leaves at once (the last item of a node's list is
+  // partial; the kernel has no workgroup barrier)
+  if ((tid & ~(kWave - 1)) >= cnt) return;
   const bool act = tid < cnt;
   const int li = off + (act ? tid : 0);
   const int i = A.tgt[li];
@@ -246,6 +249,7 @@ __global__ __launch_bounds__(kBlock) void k_far(FarArgs A) {
   const int item = blockIdx.x;
   const int node = A.item_node[item];
   const int off = A.item_off[item], cnt = A.item_cnt[item];
+  if ((tid & ~(kWave - 1)) >= cnt) return;  // a wave without targets (no workgroup barrier here)
   const bool act = tid < cnt;
   const int li = off + (act ? tid : 0);
   const int i = A.tgt[li];
