@@ -650,6 +650,9 @@ using namespace hor;
 extern "C" int hor_surface_generate_gpu(hor_ctx* ctx, const hor_body* body, double h, double pou_deg, int cap,
                                         int* n, double* x, double* nrml, double* area, int* plane) {
   if (!ctx || !body || !n || !(h > 0)) return set_error(HOR_E_ARG, "hor_surface_generate_gpu: bad argument");
+  const double rmax = std::max(body->abc[0], std::max(body->abc[1], body->abc[2]));
+  if (!(rmax > 0) || !(rmax / h < 1e6) || !(pou_deg > 0 && pou_deg <= 90))
+    return set_error(HOR_E_ARG, "hor_surface_generate_gpu: need semi-axes > 0, h above max(abc)/1e6, 0 < pou_deg <= 90");
   HOR_HIP_TRY(hipSetDevice(ctx->device));
   return surface_generate_gpu(ctx, *body, h, pou_deg, cap, n, x, nrml, area, plane);
 }
@@ -657,7 +660,8 @@ extern "C" int hor_surface_generate_gpu(hor_ctx* ctx, const hor_body* body, doub
 extern "C" int hor_nearest_data_gpu(hor_ctx* ctx, const hor_body* body, double h, double DEL, int ns,
                                     const double* sx, const int* splane, int nt, const double* tx, int* nrst,
                                     double* x0, double* n0, double* b, int* i123, double* f0) {
-  if (!ctx || !body || !sx || !splane || !tx || !nrst || !x0 || !n0 || !b || !i123 || ns <= 0 || nt < 0)
+  if (!ctx || !body || !sx || !splane || !tx || !nrst || !x0 || !n0 || !b || !i123 || ns <= 0 || nt < 0 ||
+      !(h > 0) || !(DEL > 0))
     return set_error(HOR_E_ARG, "hor_nearest_data_gpu: bad argument");
   HOR_HIP_TRY(hipSetDevice(ctx->device));
   return nearest_data_gpu(ctx, *body, h, DEL, ns, sx, splane, nt, tx, nrst, x0, n0, b, i123, f0);
