@@ -8,6 +8,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "dl_on_table.h"
 #include "erfcx_poly.h"
 #include "hor_internal.h"
 

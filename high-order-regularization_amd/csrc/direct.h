@@ -34,6 +34,7 @@ struct DirectArgs {
   const double* wave_box;          // 8 per 64-target group: lo xyz, hi xyz, all-far flag, pad
   double* partial;                 // nchunk*3*ntp partial sums
   const double* erfcx;             // kErfcxDeg+1 coefficients of the erf polynomial
+  const double* dlon_tab;          // kDlOnTable: the on-surface stresslet kernel (staged in LDS)
 };
 
 int launch_direct(const DirectArgs& A, int kind, bool mixed, int nchunk, hipStream_t s);

@@ -20,8 +20,13 @@
 
 namespace hor {
 
+// LDS bytes of a sum kernel: the on-surface stresslet table for DL_ON.
+constexpr size_t kDlOnLdsBytes = sizeof(double) * kDlOnIntervals * (kDlOnDeg + 1);
+inline size_t lds_bytes(int kind) { return kind == HOR_DL_ON ? kDlOnLdsBytes : 0; }
+
 template <int KIND, bool MIXED>
 __global__ __launch_bounds__(kBlock) void k_direct(DirectArgs A) {
+  const double* tab = stage_dl_on_table<KIND>(A);
   const int tid = threadIdx.x;
   const int i = blockIdx.x * kBlock + tid;     // sorted target (ntp is a multiple of kBlock)
   const int wave = i / kWave;
@@ -33,7 +38,7 @@ __global__ __launch_bounds__(kBlock) void k_direct(DirectArgs A) {
   // this chunk's 64-source subtiles
   const int sub0 = blockIdx.y * A.subs_per_chunk;
   const int sub1 = min(A.nsub, sub0 + A.subs_per_chunk);
-  range_sum<KIND, MIXED, true>(A, sub0 * kWave, min(A.ns, sub1 * kWave), sub0, wb, all_far, t, u);
+  range_sum<KIND, MIXED, true>(A, sub0 * kWave, min(A.ns, sub1 * kWave), sub0, wb, all_far, t, tab, u);
   double* P = A.partial + (size_t)blockIdx.y * 3 * A.ntp;
   P[i] = u[0];
   if (KIND != HOR_P_OFF) {
@@ -178,7 +183,7 @@ __global__ void k_target_f0n0_all(int nt, int ntp, const int* tperm, const doubl
 
 int launch_direct(const DirectArgs& A, int kind, bool mixed, int nchunk, hipStream_t s) {
   dim3 grid(A.ntp / kBlock, nchunk), block(kBlock);
-#define HOR_LAUNCH(K, M) hipLaunchKernelGGL((k_direct<K, M>), grid, block, 0, s, A)
+#define HOR_LAUNCH(K, M) hipLaunchKernelGGL((k_direct<K, M>), grid, block, lds_bytes(K), s, A)
   if (!mixed) {
     switch (kind) {
       case HOR_SL_ON: HOR_LAUNCH(HOR_SL_ON, false); break;

@@ -193,12 +193,13 @@ __device__ __forceinline__ double wave_max(double v) {
 // Leaf direct sums (Call_Ds_*, KITC.cpp:807-1062) for up to 256 near targets.
 template <int KIND, bool MIXED>
 __global__ __launch_bounds__(kBlock) void k_leaf(LeafArgs A) {
+  const double* tab = stage_dl_on_table<KIND>(A.D);   // the workgroup's only barrier
   const int tid = threadIdx.x;
   const int item = blockIdx.x;
   const int node = A.item_node[item];
   const int off = A.item_off[item], cnt = A.item_cnt[item];
   // a wave without targets leaves at once (the last item of a node's list is
-  // partial; the kernel has no workgroup barrier)
+  // partial)
   if ((tid & ~(kWave - 1)) >= cnt) return;
   const bool act = tid < cnt;
   const int li = off + (act ? tid : 0);
@@ -213,7 +214,7 @@ __global__ __launch_bounds__(kBlock) void k_leaf(LeafArgs A) {
   const bool all_far = off_kind && !__any(t.nearp ? 1 : 0);
   double u[3] = {0.0, 0.0, 0.0};
   const int lo = A.node_lo[node], hi = A.node_hi[node];
-  range_sum<KIND, MIXED, false>(A.D, lo, hi + 1, A.leaf_sub_off[node], wb, all_far, t, u);
+  range_sum<KIND, MIXED, false>(A.D, lo, hi + 1, A.leaf_sub_off[node], wb, all_far, t, tab, u);
   if (act) {
     int s = A.slot[li];
     A.slots[s] = u[0];
@@ -596,8 +597,9 @@ int tree_plan_create(hor_plan* P, const double* sx, const double* sn, const doub
 
 template <int KIND>
 static void launch_leaf(const LeafArgs& A, bool mixed, int nitems, hipStream_t s) {
-  if (mixed) hipLaunchKernelGGL((k_leaf<KIND, true>), dim3(nitems), dim3(kBlock), 0, s, A);
-  else hipLaunchKernelGGL((k_leaf<KIND, false>), dim3(nitems), dim3(kBlock), 0, s, A);
+  const size_t lds = KIND == HOR_DL_ON ? sizeof(double) * kDlOnIntervals * (kDlOnDeg + 1) : 0;
+  if (mixed) hipLaunchKernelGGL((k_leaf<KIND, true>), dim3(nitems), dim3(kBlock), lds, s, A);
+  else hipLaunchKernelGGL((k_leaf<KIND, false>), dim3(nitems), dim3(kBlock), lds, s, A);
 }
 
 int tree_plan_eval(hor_plan* P, const double* dens_dev, const double* d0_dev, double* out_dev, const double* base,

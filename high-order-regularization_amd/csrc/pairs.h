@@ -185,25 +185,46 @@ __device__ __forceinline__ void dl_off_full(const SrcRec& s, const Tgt& t, doubl
 }
 
 // ---- DL on-surface pair, full branch structure (stresslets.cpp:244-269).
-__device__ __forceinline__ void dl_on_full(const SrcRec& s, const Tgt& t, double inv_del,
-                                           const double* ec, double u[3]) {
+// Stages the on-surface kernel table in LDS (DL_ON only); every thread of the
+// workgroup must call it before any of them returns.
+template <int KIND>
+__device__ __forceinline__ const double* stage_dl_on_table(const DirectArgs& A) {
+  if (KIND != HOR_DL_ON) return nullptr;
+  extern __shared__ double lds_tab[];
+  for (int q = threadIdx.x; q < kDlOnIntervals * (kDlOnDeg + 1); q += blockDim.x) lds_tab[q] = A.dlon_tab[q];
+  __syncthreads();
+  return lds_tab;
+}
+
+// G(s) = F(rho)/rho^5 of the on-surface kernel from its piecewise polynomial
+// (dl_on_table.h), tab in LDS: interval k = floor(s / w), local x in [-1, 1].
+__device__ __forceinline__ double dl_on_G(const double* __restrict__ tab, double s) {
+  const int k = min((int)(s * (kDlOnIntervals / kDlOnSmax)), kDlOnIntervals - 1);
+  const double x = s * (2.0 * kDlOnIntervals / kDlOnSmax) - (double)(2 * k + 1);
+  const double* c = tab + k * (kDlOnDeg + 1);
+  double g = c[kDlOnDeg];
+#pragma unroll
+  for (int j = kDlOnDeg - 1; j >= 0; j--) g = __builtin_fma(g, x, c[j]);
+  return g;
+}
+
+// On-surface stresslet pair (Stokes_DL_7ord_onSurf, utilities.cpp:291-323):
+// H3 = F(rho)/r^5 with F = erf(rho) + p3(rho^2) rho e^{-rho^2}; within 8 delta
+// H3 = G(rho^2)/delta^5 from the table (no exp, erf or square root), beyond
+// it the plain 1/r^5.
+__device__ __forceinline__ void dl_on_full(const SrcRec& s, const Tgt& t, double inv_del2, double inv_del5,
+                                           const double* __restrict__ tab, double u[3]) {
   double dx = t.px - s.x, dy = t.py - s.y, dz = t.pz - s.z;
   double r2 = dx * dx + dy * dy + dz * dz;
   if (r2 < 1e-28) return;  // d = 0: the self term contributes exactly zero
-  double ri = rsqrt64(r2);
-  double rd = (r2 * ri) * inv_del;
-  double ri2 = ri * ri;
+  const double sq = r2 * inv_del2;
   double H3;
-  if (rd > 8.0) {
+  if (sq > 64.0) {
+    const double ri = rsqrt64(r2);
+    const double ri2 = ri * ri;
     H3 = ri2 * ri2 * ri;
   } else {
-    double rd2 = rd * rd;
-    double E = exp(-rd2);
-    double ef = erf_from_exp(rd, E, ec);
-    double rex = rd * E;
-    const double* q = t.q;
-    double p3 = q[7] + rd2 * (q[8] + rd2 * (q[9] + rd2 * q[10]));
-    H3 = (ef + p3 * rex) * (ri2 * ri2 * ri);
+    H3 = dl_on_G(tab, sq) * inv_del5;
   }
   double gx = s.d0 - t.g0x, gy = s.d1 - t.g0y, gz = s.d2 - t.g0z;
   double dxn = dx * s.nx + dy * s.ny + dz * s.nz;
@@ -418,7 +439,8 @@ __device__ __forceinline__ void load_target(const DirectArgs& A, int i, Tgt& t) 
 }
 
 // Sums over the sorted sources [j0, j1), which start at 64-subtile sub0.
-// Source records are read with wave-uniform addresses (scalar loads).
+// Source records are read with wave-uniform addresses (scalar loads).  tab is
+// the on-surface kernel's table in LDS (DL_ON only).
 //
 // Mixed precision (MIXED): a subtile beyond 8 delta of the wave runs the plain
 // kernel in FP32, on coordinates relative to the subtile's box centre (REL32,
@@ -431,9 +453,12 @@ __device__ __forceinline__ void load_target(const DirectArgs& A, int i, Tgt& t) 
 // over one subtile (at most 64 terms) and the subtile sums in FP64.
 template <int KIND, bool MIXED, bool REL32>
 __device__ __forceinline__ void range_sum(const DirectArgs& A, int j0, int j1, int sub0, const double* wb,
-                                          bool all_far, const Tgt& t, double u[3]) {
+                                          bool all_far, const Tgt& t, const double* __restrict__ tab,
+                                          double u[3]) {
   const double rp = 1.7724538509055160273;
   const double inv_del = 1.0 / A.DEL;
+  const double inv_del2 = inv_del * inv_del;
+  const double inv_del5 = inv_del2 * inv_del2 * inv_del;
   const double self1 = 32.0 / 5.0 / rp / A.DEL;
   const double self2 = 256.0 / 15.0 / rp / (A.DEL * A.DEL * A.DEL);
   const SrcRec* __restrict__ R = A.rec;
@@ -490,7 +515,7 @@ __device__ __forceinline__ void range_sum(const DirectArgs& A, int j0, int j1, i
           if (PR) p_full(s, t, inv_del, self2, A.erfcx, u);
           else if (SL) sl_full(s, t.px, t.py, t.pz, t.sub, t.nearp, t.q, inv_del, self1, self2, A.erfcx, u);
           else if (KIND == HOR_DL_OFF) dl_off_full(s, t, inv_del, self2, A.erfcx, u);
-          else dl_on_full(s, t, inv_del, A.erfcx, u);
+          else dl_on_full(s, t, inv_del2, inv_del5, tab, u);
         } else {
           if (PR) p_plain(s, t, u);
           else if (SL) sl_plain(s, t.px, t.py, t.pz, t.sub, u);

@@ -24,6 +24,7 @@ struct hor_plan {
   hor::DBuf<hor::SrcRec> rec;
   hor::DBuf<hor::SrcRec32> rec32;
   hor::DBuf<double> erfcx;   // coefficients of the erf polynomial (kErfcxC)
+  hor::DBuf<double> dlon_tab;  // on-surface stresslet table (kDlOnTable), DL_ON plans
   hor::DBuf<int> sperm;
   // targets, sorted SoA (stride ntp)
   hor::DBuf<int> tperm, tnear;
