@@ -77,3 +77,49 @@ def test_kernels_match_reference_fixture(oracle):
     assert np.array_equal(coef, g["coef"])
     for kind in ("sl", "sl_on", "dl", "dl_on"):
         assert np.array_equal(oracle.kernel(kind, g["r"], float(g["d"]), g["c"]), g[kind]), kind
+
+
+def test_on_surface_stresslet_table_matches_the_kernel(oracle):
+    """The on-surface stresslet kernel the GPU reads from its table
+    (csrc/dl_on_table.h: G(s) = F(rho)/rho^5, H3 = G/delta^5):
+    * against F(rho)/r^5 in 60-digit arithmetic (mpmath) on rho in (0, 8]:
+      2e-15 relative (a few ulps);
+    * against the oracle's Stokes_DL_7ord_onSurf (utilities.cpp:291-323) for
+      rho >= 0.5, 1e-13 relative; below that the FP64 formula itself loses
+      digits to cancellation (F ~ rho^5 from terms ~ rho), e.g. 2e-11 at
+      rho = 0.05, which the table does not."""
+    import pathlib
+    import re
+    import mpmath as mp
+    src = (pathlib.Path(__file__).parents[1] / "high-order-regularization_amd" / "csrc" / "dl_on_table.h").read_text()
+    nint = int(re.search(r"kDlOnIntervals = (\d+)", src).group(1))
+    deg = int(re.search(r"kDlOnDeg = (\d+)", src).group(1))
+    smax = float(re.search(r"kDlOnSmax = ([0-9.]+)", src).group(1))
+    body = src[src.index("kDlOnTable["):]
+    vals = [float(v) for v in re.findall(r"-?\d\.\d+(?:e[-+]\d+)?", body[body.index("{"):body.index("}")])]
+    C = np.array(vals).reshape(nint, deg + 1)
+
+    def table_H3(r, d):
+        s = (r / d) ** 2
+        k = np.minimum((s * (nint / smax)).astype(int), nint - 1)
+        x = s * (2 * nint / smax) - (2 * k + 1)
+        g = C[k, deg]
+        for j in range(deg - 1, -1, -1):
+            g = g * x + C[k, j]
+        return g / d ** 5
+
+    d = 0.1
+    mp.mp.dps = 60
+    rho = np.concatenate([np.geomspace(1e-4, 1, 60), np.linspace(1, 8, 240)])
+    exact = []
+    for p in rho:
+        q = mp.mpf(p)
+        k = 2 / mp.sqrt(mp.pi)
+        p3 = -1 - mp.mpf(2) / 3 * q ** 2 + mp.mpf(52) / 45 * q ** 4 - mp.mpf(8) / 45 * q ** 6
+        F = mp.erf(q) + k * p3 * q * mp.exp(-q * q)
+        exact.append(float(F / (q * d) ** 5))
+    exact = np.array(exact)
+    assert np.abs(table_H3(rho * d, d) / exact - 1).max() < 2e-15   # a few ulps, the /delta^5 included
+    r = np.linspace(0.5 * d, 8 * d, 600)
+    ref = oracle.kernel("dl_on", r, d)[:, 1]
+    assert np.abs(table_H3(r, d) / ref - 1).max() < 1e-13
