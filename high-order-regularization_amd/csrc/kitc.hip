@@ -81,10 +81,16 @@ __global__ void k_moment_rhs(int ns, int nsp, int dl, const double* sn, const do
   const double d[3] = {sd[p], sd[nsp + p], sd[2 * nsp + p]};
   double* o = mq + (size_t)p * 16;
   if (dl) {
-    for (int l = 0; l < 3; l++)
-      for (int m = 0; m < 3; m++) o[3 * l + m] = a * d[l] * n[m];
-    for (int m = 0; m < 3; m++) o[9 + m] = a * n[m];
-    for (int c = 12; c < 16; c++) o[c] = 0.0;
+    // d^T W_{g n} d only sees the symmetric part of W_{g n}: diagonal, then
+    // the three sums of the off-diagonal pairs, then W_n
+    o[0] = a * d[0] * n[0];
+    o[1] = a * d[1] * n[1];
+    o[2] = a * d[2] * n[2];
+    o[3] = a * (d[0] * n[1] + d[1] * n[0]);
+    o[4] = a * (d[0] * n[2] + d[2] * n[0]);
+    o[5] = a * (d[1] * n[2] + d[2] * n[1]);
+    for (int m = 0; m < 3; m++) o[6 + m] = a * n[m];
+    for (int c = 9; c < 16; c++) o[c] = 0.0;
   } else {
     for (int m = 0; m < 3; m++) {
       o[m] = a * d[m];
@@ -232,14 +238,16 @@ struct FarArgs {
 };
 
 constexpr int kNc32Sl = 8;    // FP32 moment floats per proxy (6 used)
-constexpr int kNc32Dl = 16;   // (12 used)
+constexpr int kNc32Dl = 12;   // (9 used)
 
 // Call_Treecode_SL / Call_Treecode_DL (KITC.cpp:671-777): plain Stokeslet /
 // stresslet from the (P+1)^3 proxies of one far node, with the target-side
 // subtraction folded in:
 //   SL: u += m / r + d (m.d) / r^3,             m = W_f - (f0.n0) W_n
 //   DL: u += d [d.M.d - (g0.d)(W_n.d)] / r^5,   M_lm = W_{g_l n_m}
-// (the DL form is the reference's sum_l d_l sum_m (W_lm - g0_l W_n,m) d_m).
+// (the DL form is the reference's sum_l d_l sum_m (W_lm - g0_l W_n,m) d_m;
+// d.M.d uses only the symmetric part of M, so the moments hold its diagonal
+// and the three off-diagonal sums: 9 components with W_n).
 // One work item is a far node and up to 256 of its targets; the node's
 // proxies and moments are wave-uniform.  In mixed precision the proxy
 // interactions run in FP32 on node-centred coordinates, each row of P+1
@@ -301,12 +309,12 @@ __global__ __launch_bounds__(kBlock) void k_far(FarArgs A) {
             const float d2 = qz - t3[c];
             const float ri = rsqrt32(__builtin_fmaf(d2, d2, r2ab));
             const float ri2 = ri * ri;
-            const float q0 = m[0] * d0 + m[1] * d1 + m[2] * d2;
-            const float q1 = m[3] * d0 + m[4] * d1 + m[5] * d2;
-            const float q2 = m[6] * d0 + m[7] * d1 + m[8] * d2;
-            const float wnd = m[9] * d0 + m[10] * d1 + m[11] * d2;
+            const float qa = m[0] * d0 + m[3] * d1 + m[4] * d2;
+            const float qb = m[1] * d1 + m[5] * d2;
+            const float qc = m[2] * d2;
+            const float wnd = m[6] * d0 + m[7] * d1 + m[8] * d2;
             const float gd = __builtin_fmaf(g0z, d2, gab);
-            const float Q = __builtin_fmaf(-gd, wnd, q0 * d0 + q1 * d1 + q2 * d2);
+            const float Q = __builtin_fmaf(-gd, wnd, qa * d0 + qb * d1 + qc * d2);
             const float kern = Q * (ri2 * ri2 * ri);
             v0 += d0 * kern;
             v1 += d1 * kern;
@@ -320,7 +328,7 @@ __global__ __launch_bounds__(kBlock) void k_far(FarArgs A) {
     const double* t1 = A.cheb + (size_t)3 * np1 * node;
     const double* t2 = t1 + np1;
     const double* t3 = t2 + np1;
-    constexpr int NC = DL ? 12 : 6;
+    constexpr int NC = DL ? 9 : 6;
     const double* M = A.mom + (size_t)fi * pflat * NC;
     if (!DL) {
       const double sub = A.tsub2[i];
@@ -354,12 +362,13 @@ __global__ __launch_bounds__(kBlock) void k_far(FarArgs A) {
             const double d2 = pz - t3[c];
             const double ri = rsqrt64(__builtin_fma(d2, d2, r2ab));
             const double ri2 = ri * ri;
-            const double q0 = m[0] * d0 + m[1] * d1 + m[2] * d2;
-            const double q1 = m[3] * d0 + m[4] * d1 + m[5] * d2;
-            const double q2 = m[6] * d0 + m[7] * d1 + m[8] * d2;
-            const double wnd = m[9] * d0 + m[10] * d1 + m[11] * d2;
+            // d^T M d = d0 (M00 d0 + S01 d1 + S02 d2) + d1 (M11 d1 + S12 d2) + d2 M22 d2
+            const double qa = m[0] * d0 + m[3] * d1 + m[4] * d2;
+            const double qb = m[1] * d1 + m[5] * d2;
+            const double qc = m[2] * d2;
+            const double wnd = m[6] * d0 + m[7] * d1 + m[8] * d2;
             const double gd = __builtin_fma(g0z, d2, gab);
-            const double Q = __builtin_fma(-gd, wnd, q0 * d0 + q1 * d1 + q2 * d2);
+            const double Q = __builtin_fma(-gd, wnd, qa * d0 + qb * d1 + qc * d2);
             const double kern = Q * (ri2 * ri2 * ri);
             u0 += d0 * kern;
             u1 += d1 * kern;
@@ -447,7 +456,7 @@ int tree_plan_create(hor_plan* P, const double* sx, const double* sn, const doub
   TP.np1 = tp.P + 1;
   TP.pflat = TP.np1 * TP.np1 * TP.np1;
   bool dl = (P->kind == HOR_DL_ON || P->kind == HOR_DL_OFF);
-  TP.ncomp = dl ? 12 : 6;
+  TP.ncomp = dl ? 9 : 6;
   // sources in tree order; targets sorted spatially (the class rule of the
   // treecode's leaf sums: |b|/delta > 8, KITC.cpp:856, 970)
   HOR_TRY(setup_sources(P, T.perm, sx, sn, sa));
