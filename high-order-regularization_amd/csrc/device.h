@@ -9,7 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include "dl_on_table.h"
-#include "erfcx_poly.h"
+#include "erf_exp_table.h"
 #include "hor_internal.h"
 
 #define HOR_HIP_TRY(expr)                                                                  \
@@ -134,26 +134,12 @@ __device__ __forceinline__ double rsqrt64(double x) {
 
 __device__ __forceinline__ float rsqrt32(float x) { return __builtin_amdgcn_rsqf(x); }
 
-// erf(rho) for 0 <= rho <= 8 given E = exp(-rho^2):  erf = 1 - E R(t) with
-// R(t) the degree-18 polynomial of erfcx(rho) = erfc(rho) exp(rho^2) in
-// t = 1/(1 + rho/3) (coefficients C = kErfcxC of erfcx_poly.h; absolute
-// error below 7e-16 in FP64).
-// Sharing E with the regularization's rho exp(-rho^2) term leaves one
-// exponential per pair.  1/(1 + rho/3): hardware reciprocal refined by one
-// third-order step t <- t (1 + e + e^2), e = 1 - d t.
-__device__ __forceinline__ double erf_from_exp(double rho, double E, const double* __restrict__ C) {
-  const double d = __builtin_fma(rho, kErfcxKappaInv, 1.0);
-  double t = __builtin_amdgcn_rcp(d);
-  const double e = __builtin_fma(-d, t, 1.0);
-  t = __builtin_fma(t * e, 1.0 + e, t);
-  const double u = t - kErfcxTmid;
-  // Estrin's scheme: pairs C[2i] + C[2i+1] u, then Horner in u^2 over the pairs
-  const double u2 = u * u;
-  double R = C[kErfcxDeg];
-#pragma unroll
-  for (int i = kErfcxDeg / 2 - 1; i >= 0; i--) R = __builtin_fma(R, u2, __builtin_fma(C[2 * i + 1], u, C[2 * i]));
-  return __builtin_fma(-E, R, 1.0);
-}
+
+// LDS doubles of a sum kernel's table: the on-surface stresslet's G for
+// DL_ON, erf(rho)/rho and exp(-rho^2) for every other kind.
+constexpr int kDlOnTabLen = kDlOnIntervals * (kDlOnDeg + 1);
+constexpr int kAETabLen = kAEIntervals * 2 * (kAEDeg + 1);
+__host__ __device__ inline int table_len(int kind) { return kind == HOR_DL_ON ? kDlOnTabLen : kAETabLen; }
 
 // Source record read by the sum kernels, one per sorted source: position,
 // weight a, a-scaled normal a n, and the density, a-scaled for the single layer

@@ -33,8 +33,8 @@ struct DirectArgs {
   const int* tnear;                // ntp: 1 = regularized target (|b|/delta <= 8 or on surface)
   const double* wave_box;          // 8 per 64-target group: lo xyz, hi xyz, all-far flag, pad
   double* partial;                 // nchunk*3*ntp partial sums
-  const double* erfcx;             // kErfcxDeg+1 coefficients of the erf polynomial
   const double* dlon_tab;          // kDlOnTable: the on-surface stresslet kernel (staged in LDS)
+  const double* ae_tab;            // kErfExpTable: erf(rho)/rho and exp(-rho^2) (staged in LDS)
 };
 
 int launch_direct(const DirectArgs& A, int kind, bool mixed, int nchunk, hipStream_t s);

@@ -20,13 +20,12 @@
 
 namespace hor {
 
-// LDS bytes of a sum kernel: the on-surface stresslet table for DL_ON.
-constexpr size_t kDlOnLdsBytes = sizeof(double) * kDlOnIntervals * (kDlOnDeg + 1);
-inline size_t lds_bytes(int kind) { return kind == HOR_DL_ON ? kDlOnLdsBytes : 0; }
+// LDS bytes of a sum kernel: its table (stage_table).
+inline size_t lds_bytes(int kind) { return sizeof(double) * table_len(kind); }
 
 template <int KIND, bool MIXED>
 __global__ __launch_bounds__(kBlock) void k_direct(DirectArgs A) {
-  const double* tab = stage_dl_on_table<KIND>(A);
+  const double* tab = stage_table<KIND>(A);
   const int tid = threadIdx.x;
   const int i = blockIdx.x * kBlock + tid;     // sorted target (ntp is a multiple of kBlock)
   const int wave = i / kWave;

@@ -199,7 +199,7 @@ __device__ __forceinline__ double wave_max(double v) {
 // Leaf direct sums (Call_Ds_*, KITC.cpp:807-1062) for up to 256 near targets.
 template <int KIND, bool MIXED>
 __global__ __launch_bounds__(kBlock) void k_leaf(LeafArgs A) {
-  const double* tab = stage_dl_on_table<KIND>(A.D);   // the workgroup's only barrier
+  const double* tab = stage_table<KIND>(A.D);   // the workgroup's only barrier
   const int tid = threadIdx.x;
   const int item = blockIdx.x;
   const int node = A.item_node[item];
@@ -606,7 +606,7 @@ int tree_plan_create(hor_plan* P, const double* sx, const double* sn, const doub
 
 template <int KIND>
 static void launch_leaf(const LeafArgs& A, bool mixed, int nitems, hipStream_t s) {
-  const size_t lds = KIND == HOR_DL_ON ? sizeof(double) * kDlOnIntervals * (kDlOnDeg + 1) : 0;
+  const size_t lds = sizeof(double) * table_len(KIND);
   if (mixed) hipLaunchKernelGGL((k_leaf<KIND, true>), dim3(nitems), dim3(kBlock), lds, s, A);
   else hipLaunchKernelGGL((k_leaf<KIND, false>), dim3(nitems), dim3(kBlock), lds, s, A);
 }

@@ -10,6 +10,46 @@
 
 namespace hor {
 
+// Stages the kind's table in LDS; every thread of the workgroup must call it
+// before any of them returns.
+template <int KIND>
+__device__ __forceinline__ const double* stage_table(const DirectArgs& A) {
+  extern __shared__ double lds_tab[];
+  const double* src = KIND == HOR_DL_ON ? A.dlon_tab : A.ae_tab;
+  for (int q = threadIdx.x; q < table_len(KIND); q += blockDim.x) lds_tab[q] = src[q];
+  __syncthreads();
+  return lds_tab;
+}
+
+// A = erf(rho)/rho and E = exp(-rho^2) at s = rho^2 <= 64 from the LDS table
+// (erf_exp_table.h): interval k = floor(s / w), local x in [-1, 1].
+__device__ __forceinline__ void erf_exp_AE(const double* __restrict__ tab, double s, double& A, double& E) {
+  const int k = min((int)(s * (kAEIntervals / kAESmax)), kAEIntervals - 1);
+  const double x = s * (2.0 * kAEIntervals / kAESmax) - (double)(2 * k + 1);
+  const double* c = tab + k * 2 * (kAEDeg + 1);
+  double a = c[kAEDeg], e = c[2 * kAEDeg + 1];
+#pragma unroll
+  for (int j = kAEDeg - 1; j >= 0; j--) {
+    a = __builtin_fma(a, x, c[j]);
+    e = __builtin_fma(e, x, c[kAEDeg + 1 + j]);
+  }
+  A = a;
+  E = e;
+}
+
+// G(s) = F(rho)/rho^5 of the on-surface kernel from its piecewise polynomial
+// (dl_on_table.h), tab in LDS: interval k = floor(s / w), local x in [-1, 1].
+__device__ __forceinline__ double dl_on_G(const double* __restrict__ tab, double s) {
+  const int k = min((int)(s * (kDlOnIntervals / kDlOnSmax)), kDlOnIntervals - 1);
+  const double x = s * (2.0 * kDlOnIntervals / kDlOnSmax) - (double)(2 * k + 1);
+  const double* c = tab + k * (kDlOnDeg + 1);
+  double g = c[kDlOnDeg];
+#pragma unroll
+  for (int j = kDlOnDeg - 1; j >= 0; j--) g = __builtin_fma(g, x, c[j]);
+  return g;
+}
+
+
 // Box distance test between a wave's target box and a source subtile box.
 __device__ __forceinline__ bool boxes_far(const double* wb, const double* sb, double far_d2) {
   double d2 = 0.0;
@@ -63,8 +103,8 @@ __device__ __forceinline__ void sl_plain32(const SrcRec32& s, float dx, float dy
 // (utilities.cpp:179-207, stokeslets.cpp:106-130).  r = sqrt(r2) and 1/r come
 // from one reciprocal square root; the self test r < 1e-14 is r2 < 1e-28.
 __device__ __forceinline__ void sl_full(const SrcRec& s, double px, double py, double pz,
-                                        double sub, bool nearp, const double* q, double inv_del,
-                                        double self1, double self2, const double* ec, double u[3]) {
+                                        double sub, bool nearp, const double* q, double inv_del, double inv_del2,
+                                        double self1, double self2, const double* __restrict__ tab, double u[3]) {
   double dx = px - s.x, dy = py - s.y, dz = pz - s.z;
   double r2 = dx * dx + dy * dy + dz * dz;
   double fx = s.d0 - sub * s.nx, fy = s.d1 - sub * s.ny, fz = s.d2 - sub * s.nz;
@@ -74,21 +114,19 @@ __device__ __forceinline__ void sl_full(const SrcRec& s, double px, double py, d
     s1 = self1;
     s2 = self2;
   } else {
-    double ri = rsqrt64(r2);
-    double rd = (r2 * ri) * inv_del;
-    double ri3 = ri * ri * ri;
-    if (!nearp || rd > 8.0) {
+    const double ri = rsqrt64(r2);
+    const double sq = r2 * inv_del2;   // rho^2
+    if (!nearp || sq > 64.0) {
       s1 = ri;
-      s2 = ri3;
+      s2 = ri * ri * ri;
     } else {
-      double rd2 = rd * rd;
-      double E = exp(-rd2);
-      double ef = erf_from_exp(rd, E, ec);
-      double rex = rd * E;
-      double p1 = q[0] + rd2 * (q[1] + rd2 * q[2]);
-      double p2 = q[3] + rd2 * (q[4] + rd2 * (q[5] + rd2 * q[6]));
-      s1 = (ef + p1 * rex) * ri;
-      s2 = (ef + p2 * rex) * ri3;
+      // erf(rho) = rho A, rho e^{-rho^2} = rho E and rho / r = 1/delta
+      double A, E;
+      erf_exp_AE(tab, sq, A, E);
+      const double p1 = q[0] + sq * (q[1] + sq * q[2]);
+      const double p2 = q[3] + sq * (q[4] + sq * (q[5] + sq * q[6]));
+      s1 = (A + p1 * E) * inv_del;
+      s2 = (A + p2 * E) * inv_del * (ri * ri);
     }
   }
   double H2 = s2 * fdx;
@@ -132,8 +170,8 @@ __device__ __forceinline__ void dl_plain32(const SrcRec32& s, float dx, float dy
 // ---- DL off-surface pair, full branch structure (stresslets.cpp:117-168).
 // Every term is linear in the source normal, so the record's a n carries the
 // weight.
-__device__ __forceinline__ void dl_off_full(const SrcRec& s, const Tgt& t, double inv_del,
-                                            double self2, const double* ec, double u[3]) {
+__device__ __forceinline__ void dl_off_full(const SrcRec& s, const Tgt& t, double inv_del, double inv_del2,
+                                            double self2, const double* __restrict__ tab, double u[3]) {
   double dx = t.px - s.x, dy = t.py - s.y, dz = t.pz - s.z;
   double r2 = dx * dx + dy * dy + dz * dz;
   double gx = s.d0 - t.g0x, gy = s.d1 - t.g0y, gz = s.d2 - t.g0z;
@@ -143,9 +181,9 @@ __device__ __forceinline__ void dl_off_full(const SrcRec& s, const Tgt& t, doubl
     H3 = 0.0;
   } else {
     double ri = rsqrt64(r2);
-    double rd = (r2 * ri) * inv_del;
     double ri2 = ri * ri;
-    if (!t.nearp || rd > 8.0) {
+    const double sq = r2 * inv_del2;   // rho^2
+    if (!t.nearp || sq > 64.0) {
       double gdx = gx * dx + gy * dy + gz * dz;
       double ndx = s.nx * dx + s.ny * dy + s.nz * dz;
       double k = gdx * ndx * (ri2 * ri2 * ri);
@@ -154,16 +192,15 @@ __device__ __forceinline__ void dl_off_full(const SrcRec& s, const Tgt& t, doubl
       u[2] += dz * k;
       return;
     }
-    double ri3 = ri2 * ri;
-    double rd2 = rd * rd;
-    double E = exp(-rd2);
-    double ef = erf_from_exp(rd, E, ec);
-    double rex = rd * E;
+    // erf(rho) = rho A, rho e^{-rho^2} = rho E and rho / r = 1/delta
+    double A, E;
+    erf_exp_AE(tab, sq, A, E);
     const double* q = t.q;
-    double p2 = q[3] + rd2 * (q[4] + rd2 * (q[5] + rd2 * q[6]));
-    double p3 = q[7] + rd2 * (q[8] + rd2 * (q[9] + rd2 * (q[10] + rd2 * q[11])));
-    H2 = (ef + p2 * rex) * ri3;
-    H3 = (ef + p3 * rex) * (ri3 * ri2);
+    double p2 = q[3] + sq * (q[4] + sq * (q[5] + sq * q[6]));
+    double p3 = q[7] + sq * (q[8] + sq * (q[9] + sq * (q[10] + sq * q[11])));
+    const double w2 = inv_del * ri2;
+    H2 = (A + p2 * E) * w2;
+    H3 = (A + p3 * E) * (w2 * ri2);
   }
   double xh0 = s.x - t.x0x, xh1 = s.y - t.x0y, xh2 = s.z - t.x0z;
   double sig2 = (xh0 * xh0 + xh1 * xh1 + xh2 * xh2) - 2.0 * t.b * (xh0 * t.n0x + xh1 * t.n0y + xh2 * t.n0z);
@@ -185,29 +222,6 @@ __device__ __forceinline__ void dl_off_full(const SrcRec& s, const Tgt& t, doubl
 }
 
 // ---- DL on-surface pair, full branch structure (stresslets.cpp:244-269).
-// Stages the on-surface kernel table in LDS (DL_ON only); every thread of the
-// workgroup must call it before any of them returns.
-template <int KIND>
-__device__ __forceinline__ const double* stage_dl_on_table(const DirectArgs& A) {
-  if (KIND != HOR_DL_ON) return nullptr;
-  extern __shared__ double lds_tab[];
-  for (int q = threadIdx.x; q < kDlOnIntervals * (kDlOnDeg + 1); q += blockDim.x) lds_tab[q] = A.dlon_tab[q];
-  __syncthreads();
-  return lds_tab;
-}
-
-// G(s) = F(rho)/rho^5 of the on-surface kernel from its piecewise polynomial
-// (dl_on_table.h), tab in LDS: interval k = floor(s / w), local x in [-1, 1].
-__device__ __forceinline__ double dl_on_G(const double* __restrict__ tab, double s) {
-  const int k = min((int)(s * (kDlOnIntervals / kDlOnSmax)), kDlOnIntervals - 1);
-  const double x = s * (2.0 * kDlOnIntervals / kDlOnSmax) - (double)(2 * k + 1);
-  const double* c = tab + k * (kDlOnDeg + 1);
-  double g = c[kDlOnDeg];
-#pragma unroll
-  for (int j = kDlOnDeg - 1; j >= 0; j--) g = __builtin_fma(g, x, c[j]);
-  return g;
-}
-
 // On-surface stresslet pair (Stokes_DL_7ord_onSurf, utilities.cpp:291-323):
 // H3 = F(rho)/r^5 with F = erf(rho) + p3(rho^2) rho e^{-rho^2}; within 8 delta
 // H3 = G(rho^2)/delta^5 from the table (no exp, erf or square root), beyond
@@ -383,8 +397,8 @@ __device__ __forceinline__ void p_plain(const SrcRec& s, const Tgt& t, double u[
 // Full branch structure of Lapl_DL_7ord (extend_to_grid/utilities.cpp:190-211):
 // self limit, plain, or (erf + P2(rho^2) rho e^{-rho^2}) / r^3 with the
 // Stokes_SL_7ord s2 polynomial (the same coefficients q[3..6]).
-__device__ __forceinline__ void p_full(const SrcRec& s, const Tgt& t, double inv_del, double self2,
-                                       const double* ec, double u[3]) {
+__device__ __forceinline__ void p_full(const SrcRec& s, const Tgt& t, double inv_del, double inv_del2,
+                                       double self2, const double* __restrict__ tab, double u[3]) {
   const double dx = t.px - s.x, dy = t.py - s.y, dz = t.pz - s.z;
   const double r2 = dx * dx + dy * dy + dz * dz;
   double H2;
@@ -392,16 +406,14 @@ __device__ __forceinline__ void p_full(const SrcRec& s, const Tgt& t, double inv
     H2 = self2;
   } else {
     const double ri = rsqrt64(r2);
-    const double rd = (r2 * ri) * inv_del;
-    const double ri3 = ri * ri * ri;
-    if (!t.nearp || rd > 8.0) {
-      H2 = ri3;
+    const double sq = r2 * inv_del2;   // rho^2
+    if (!t.nearp || sq > 64.0) {
+      H2 = ri * ri * ri;
     } else {
-      const double rd2 = rd * rd;
-      const double E = exp(-rd2);
-      const double ef = erf_from_exp(rd, E, ec);
-      const double p2 = t.q[3] + rd2 * (t.q[4] + rd2 * (t.q[5] + rd2 * t.q[6]));
-      H2 = (ef + p2 * (rd * E)) * ri3;
+      double A, E;
+      erf_exp_AE(tab, sq, A, E);
+      const double p2 = t.q[3] + sq * (t.q[4] + sq * (t.q[5] + sq * t.q[6]));
+      H2 = (A + p2 * E) * (inv_del * (ri * ri));
     }
   }
   u[0] += p_numer(s, t, dx, dy, dz) * H2;
@@ -440,7 +452,7 @@ __device__ __forceinline__ void load_target(const DirectArgs& A, int i, Tgt& t) 
 
 // Sums over the sorted sources [j0, j1), which start at 64-subtile sub0.
 // Source records are read with wave-uniform addresses (scalar loads).  tab is
-// the on-surface kernel's table in LDS (DL_ON only).
+// the kind's table in LDS (stage_table).
 //
 // Mixed precision (MIXED): a subtile beyond 8 delta of the wave runs the plain
 // kernel in FP32, on coordinates relative to the subtile's box centre (REL32,
@@ -512,9 +524,9 @@ __device__ __forceinline__ void range_sum(const DirectArgs& A, int j0, int j1, i
           else if (KIND == HOR_DL_OFF) dl_off_reg32(s32, fx, fy, fz, v, t.nearp, inv_del32, w);
           else dl_on_reg32(s32, fx, fy, fz, v, inv_del32, w);
         } else if (__any(need ? 1 : 0)) {
-          if (PR) p_full(s, t, inv_del, self2, A.erfcx, u);
-          else if (SL) sl_full(s, t.px, t.py, t.pz, t.sub, t.nearp, t.q, inv_del, self1, self2, A.erfcx, u);
-          else if (KIND == HOR_DL_OFF) dl_off_full(s, t, inv_del, self2, A.erfcx, u);
+          if (PR) p_full(s, t, inv_del, inv_del2, self2, tab, u);
+          else if (SL) sl_full(s, t.px, t.py, t.pz, t.sub, t.nearp, t.q, inv_del, inv_del2, self1, self2, tab, u);
+          else if (KIND == HOR_DL_OFF) dl_off_full(s, t, inv_del, inv_del2, self2, tab, u);
           else dl_on_full(s, t, inv_del2, inv_del5, tab, u);
         } else {
           if (PR) p_plain(s, t, u);

@@ -230,8 +230,8 @@ int setup_sources(hor_plan* P, const std::vector<int>& perm, const double* sx, c
   int nsp = std::max(kWave, round_up(ns, kWave));
   P->nsp = nsp;
   P->sperm_h = perm;
-  HOR_TRY(P->erfcx.upload(kErfcxC, kErfcxDeg + 1, s));
-  if (P->kind == HOR_DL_ON) HOR_TRY(P->dlon_tab.upload(kDlOnTable, kDlOnIntervals * (kDlOnDeg + 1), s));
+  if (P->kind == HOR_DL_ON) HOR_TRY(P->dlon_tab.upload(kDlOnTable, kDlOnTabLen, s));
+  else HOR_TRY(P->ae_tab.upload(kErfExpTable, kAETabLen, s));
   std::vector<double> X(3 * (size_t)nsp, 0.0), N(3 * (size_t)nsp, 0.0), A((size_t)nsp, 0.0);
   for (int j = 0; j < ns; j++) {
     int o = perm[j];
@@ -281,8 +281,8 @@ DirectArgs direct_args(hor_plan* P) {
   A.tx = P->tx.p; A.tcoef = P->tcoef.p; A.tsub = P->tsub.p;
   A.tx0 = P->tx0.p; A.tn0 = P->tn0.p; A.tb = P->tb.p;
   A.tnear = P->tnear.p; A.wave_box = P->wave_box.p; A.partial = P->partial.p;
-  A.erfcx = P->erfcx.p;
   A.dlon_tab = P->dlon_tab.p;
+  A.ae_tab = P->ae_tab.p;
   return A;
 }
 
