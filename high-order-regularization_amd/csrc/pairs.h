@@ -462,7 +462,9 @@ __device__ __forceinline__ void load_target(const DirectArgs& A, int i, Tgt& t) 
 // regularized code if some lane is within 8 delta (d from the FP64
 // difference, so no cancellation), else to the FP64 plain kernel (which
 // reuses the vote's FP64 difference).  FP32 contributions are summed in FP32
-// over one subtile (at most 64 terms) and the subtile sums in FP64.
+// over one subtile (at most 64 terms) and the subtile sums in FP64.  The
+// on-surface stresslet keeps every regularized pair in FP64: its tabulated
+// kernel costs less than the FP32 formula.
 template <int KIND, bool MIXED, bool REL32>
 __device__ __forceinline__ void range_sum(const DirectArgs& A, int j0, int j1, int sub0, const double* wb,
                                           bool all_far, const Tgt& t, const double* __restrict__ tab,
@@ -517,7 +519,8 @@ __device__ __forceinline__ void range_sum(const DirectArgs& A, int j0, int j1, i
         const double dx = t.px - s.x, dy = t.py - s.y, dz = t.pz - s.z;
         const double r2 = dx * dx + dy * dy + dz * dz;
         const bool need = t.nearp && !(r2 > A.near_r2);
-        if (MIXED && !__any((t.nearp && !(r2 > A.near64_r2)) ? 1 : 0) && __any(need ? 1 : 0)) {
+        if (MIXED && KIND != HOR_DL_ON && !__any((t.nearp && !(r2 > A.near64_r2)) ? 1 : 0) &&
+            __any(need ? 1 : 0)) {
           const SrcRec32& s32 = R32[j];
           const float fx = (float)dx, fy = (float)dy, fz = (float)dz;
           if (SL) sl_reg32(s32, fx, fy, fz, v, t.nearp, inv_del32, w);
