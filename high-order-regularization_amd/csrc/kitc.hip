@@ -282,6 +282,7 @@ __global__ __launch_bounds__(kBlock) void k_far(FarArgs A) {
           const float d1 = qy - t2[b];
           const float r2ab = d0 * d0 + d1 * d1;
           float v0 = 0.f, v1 = 0.f, v2 = 0.f;
+#pragma unroll 2
           for (int c = 0; c < np1; c++, kk++) {
             const float* m = M + (size_t)kk * NC;
             const float d2 = qz - t3[c];
@@ -299,28 +300,29 @@ __global__ __launch_bounds__(kBlock) void k_far(FarArgs A) {
       const float g0x = (float)A.tsub2[i], g0y = (float)A.tsub2[A.ntp + i], g0z = (float)A.tsub2[2 * A.ntp + i];
       for (int a = 0, kk = 0; a < np1; a++) {
         const float d0 = qx - t1[a];
+        const float d00 = d0 * d0;
         for (int b = 0; b < np1; b++) {
           const float d1 = qy - t2[b];
-          const float r2ab = d0 * d0 + d1 * d1;
+          const float r2ab = __builtin_fmaf(d1, d1, d00);
           const float gab = g0x * d0 + g0y * d1;
-          float v0 = 0.f, v1 = 0.f, v2 = 0.f;
+          const float d11 = d1 * d1, d01 = d0 * d1;
+          float kab = 0.f, v2 = 0.f;
+#pragma unroll 2
           for (int c = 0; c < np1; c++, kk++) {
             const float* m = M + (size_t)kk * NC;
             const float d2 = qz - t3[c];
             const float ri = rsqrt32(__builtin_fmaf(d2, d2, r2ab));
             const float ri2 = ri * ri;
-            const float qa = m[0] * d0 + m[3] * d1 + m[4] * d2;
-            const float qb = m[1] * d1 + m[5] * d2;
-            const float qc = m[2] * d2;
+            const float qab = __builtin_fmaf(m[3], d01, __builtin_fmaf(m[1], d11, m[0] * d00));
+            const float w = __builtin_fmaf(m[2], d2, __builtin_fmaf(m[5], d1, m[4] * d0));
             const float wnd = m[6] * d0 + m[7] * d1 + m[8] * d2;
             const float gd = __builtin_fmaf(g0z, d2, gab);
-            const float Q = __builtin_fmaf(-gd, wnd, qa * d0 + qb * d1 + qc * d2);
+            const float Q = __builtin_fmaf(-gd, wnd, __builtin_fmaf(w, d2, qab));
             const float kern = Q * (ri2 * ri2 * ri);
-            v0 += d0 * kern;
-            v1 += d1 * kern;
+            kab += kern;
             v2 += d2 * kern;
-                      }
-          u0 += (double)v0; u1 += (double)v1; u2 += (double)v2;
+          }
+          u0 += (double)(d0 * kab); u1 += (double)(d1 * kab); u2 += (double)v2;
         }
       }
     }
@@ -337,6 +339,7 @@ __global__ __launch_bounds__(kBlock) void k_far(FarArgs A) {
         for (int b = 0; b < np1; b++) {
           const double d1 = py - t2[b];
           const double r2ab = d0 * d0 + d1 * d1;
+#pragma unroll 2
           for (int c = 0; c < np1; c++, kk++) {
             const double* m = M + (size_t)kk * NC;
             const double d2 = pz - t3[c];
@@ -353,27 +356,31 @@ __global__ __launch_bounds__(kBlock) void k_far(FarArgs A) {
       const double g0x = A.tsub2[i], g0y = A.tsub2[A.ntp + i], g0z = A.tsub2[2 * A.ntp + i];
       for (int a = 0, kk = 0; a < np1; a++) {
         const double d0 = px - t1[a];
+        const double d00 = d0 * d0;
         for (int b = 0; b < np1; b++) {
           const double d1 = py - t2[b];
-          const double r2ab = d0 * d0 + d1 * d1;
+          const double r2ab = __builtin_fma(d1, d1, d00);
           const double gab = g0x * d0 + g0y * d1;
+          const double d11 = d1 * d1, d01 = d0 * d1;
+          double kab = 0.0;  // the row's kernel sum: d0 and d1 are fixed along it
+#pragma unroll 2
           for (int c = 0; c < np1; c++, kk++) {
             const double* m = M + (size_t)kk * NC;
             const double d2 = pz - t3[c];
             const double ri = rsqrt64(__builtin_fma(d2, d2, r2ab));
             const double ri2 = ri * ri;
-            // d^T M d = d0 (M00 d0 + S01 d1 + S02 d2) + d1 (M11 d1 + S12 d2) + d2 M22 d2
-            const double qa = m[0] * d0 + m[3] * d1 + m[4] * d2;
-            const double qb = m[1] * d1 + m[5] * d2;
-            const double qc = m[2] * d2;
+            // d^T M d = M00 d0^2 + M11 d1^2 + S01 d0 d1 + d2 (S02 d0 + S12 d1 + M22 d2)
+            const double qab = __builtin_fma(m[3], d01, __builtin_fma(m[1], d11, m[0] * d00));
+            const double w = __builtin_fma(m[2], d2, __builtin_fma(m[5], d1, m[4] * d0));
             const double wnd = m[6] * d0 + m[7] * d1 + m[8] * d2;
             const double gd = __builtin_fma(g0z, d2, gab);
-            const double Q = __builtin_fma(-gd, wnd, qa * d0 + qb * d1 + qc * d2);
+            const double Q = __builtin_fma(-gd, wnd, __builtin_fma(w, d2, qab));
             const double kern = Q * (ri2 * ri2 * ri);
-            u0 += d0 * kern;
-            u1 += d1 * kern;
+            kab += kern;
             u2 += d2 * kern;
           }
+          u0 += d0 * kab;
+          u1 += d1 * kab;
         }
       }
     }

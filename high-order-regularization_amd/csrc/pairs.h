@@ -175,7 +175,8 @@ __device__ __forceinline__ void dl_off_full(const SrcRec& s, const Tgt& t, doubl
   double dx = t.px - s.x, dy = t.py - s.y, dz = t.pz - s.z;
   double r2 = dx * dx + dy * dy + dz * dz;
   double gx = s.d0 - t.g0x, gy = s.d1 - t.g0y, gz = s.d2 - t.g0z;
-  double H2, H3;
+  double H2 = 0.0, H3;
+  bool plain = false;
   if (t.nearp && r2 < 1e-28) {
     H2 = self2;
     H3 = 0.0;
@@ -184,41 +185,50 @@ __device__ __forceinline__ void dl_off_full(const SrcRec& s, const Tgt& t, doubl
     double ri2 = ri * ri;
     const double sq = r2 * inv_del2;   // rho^2
     if (!t.nearp || sq > 64.0) {
-      double gdx = gx * dx + gy * dy + gz * dz;
-      double ndx = s.nx * dx + s.ny * dy + s.nz * dz;
-      double k = gdx * ndx * (ri2 * ri2 * ri);
-      u[0] += dx * k;
-      u[1] += dy * k;
-      u[2] += dz * k;
-      return;
+      plain = true;
+      H3 = ri2 * ri2 * ri;
+    } else {
+      // erf(rho) = rho A, rho e^{-rho^2} = rho E and rho / r = 1/delta
+      double A, E;
+      erf_exp_AE(tab, sq, A, E);
+      const double* q = t.q;
+      double p2 = q[3] + sq * (q[4] + sq * (q[5] + sq * q[6]));
+      double p3 = q[7] + sq * (q[8] + sq * (q[9] + sq * (q[10] + sq * q[11])));
+      const double w2 = inv_del * ri2;
+      H2 = (A + p2 * E) * w2;
+      H3 = (A + p3 * E) * (w2 * ri2);
     }
-    // erf(rho) = rho A, rho e^{-rho^2} = rho E and rho / r = 1/delta
-    double A, E;
-    erf_exp_AE(tab, sq, A, E);
-    const double* q = t.q;
-    double p2 = q[3] + sq * (q[4] + sq * (q[5] + sq * q[6]));
-    double p3 = q[7] + sq * (q[8] + sq * (q[9] + sq * (q[10] + sq * q[11])));
-    const double w2 = inv_del * ri2;
-    H2 = (A + p2 * E) * w2;
-    H3 = (A + p3 * E) * (w2 * ri2);
   }
-  double xh0 = s.x - t.x0x, xh1 = s.y - t.x0y, xh2 = s.z - t.x0z;
-  double sig2 = (xh0 * xh0 + xh1 * xh1 + xh2 * xh2) - 2.0 * t.b * (xh0 * t.n0x + xh1 * t.n0y + xh2 * t.n0z);
-  double g_n0 = gx * t.n0x + gy * t.n0y + gz * t.n0z;
-  double g_xh = gx * xh0 + gy * xh1 + gz * xh2;
-  double n0_n = t.n0x * s.nx + t.n0y * s.ny + t.n0z * s.nz;
-  double xh_n = xh0 * s.nx + xh1 * s.ny + xh2 * s.nz;
-  double m1a = g_n0 * n0_n;
-  double m1b = g_xh * n0_n + g_n0 * xh_n;
-  double m2a = g_xh * xh_n;
-  double m2b = t.b * m1b;
-  // t1 = d m1a - n0 m1b;  u += t1 (H2 - sig2 H3) + (d m2a + xh m2b) H3,
-  // collected per vector: u += d Bd - n0 Bn + xh Bx
-  double A1 = H2 - sig2 * H3;
-  double Bd = m1a * A1 + m2a * H3, Bn = m1b * A1, Bx = m2b * H3;
-  u[0] += dx * Bd - t.n0x * Bn + xh0 * Bx;
-  u[1] += dy * Bd - t.n0y * Bn + xh1 * Bx;
-  u[2] += dz * Bd - t.n0z * Bn + xh2 * Bx;
+  double c0, c1, c2;
+  if (plain) {
+    double gdx = gx * dx + gy * dy + gz * dz;
+    double ndx = s.nx * dx + s.ny * dy + s.nz * dz;
+    double k = gdx * ndx * H3;
+    c0 = dx * k;
+    c1 = dy * k;
+    c2 = dz * k;
+  } else {
+    double xh0 = s.x - t.x0x, xh1 = s.y - t.x0y, xh2 = s.z - t.x0z;
+    double sig2 = (xh0 * xh0 + xh1 * xh1 + xh2 * xh2) - 2.0 * t.b * (xh0 * t.n0x + xh1 * t.n0y + xh2 * t.n0z);
+    double g_n0 = gx * t.n0x + gy * t.n0y + gz * t.n0z;
+    double g_xh = gx * xh0 + gy * xh1 + gz * xh2;
+    double n0_n = t.n0x * s.nx + t.n0y * s.ny + t.n0z * s.nz;
+    double xh_n = xh0 * s.nx + xh1 * s.ny + xh2 * s.nz;
+    double m1a = g_n0 * n0_n;
+    double m1b = g_xh * n0_n + g_n0 * xh_n;
+    double m2a = g_xh * xh_n;
+    double m2b = t.b * m1b;
+    // t1 = d m1a - n0 m1b;  u += t1 (H2 - sig2 H3) + (d m2a + xh m2b) H3,
+    // collected per vector: u += d Bd - n0 Bn + xh Bx
+    double A1 = H2 - sig2 * H3;
+    double Bd = m1a * A1 + m2a * H3, Bn = m1b * A1, Bx = m2b * H3;
+    c0 = dx * Bd - t.n0x * Bn + xh0 * Bx;
+    c1 = dy * Bd - t.n0y * Bn + xh1 * Bx;
+    c2 = dz * Bd - t.n0z * Bn + xh2 * Bx;
+  }
+  u[0] += c0;
+  u[1] += c1;
+  u[2] += c2;
 }
 
 // ---- DL on-surface pair, full branch structure (stresslets.cpp:244-269).
