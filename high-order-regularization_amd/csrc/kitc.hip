@@ -53,6 +53,11 @@ struct TreePlan {
   DBuf<float> mom32;             // mixed precision: [kk][ncomp32] FP32 copy (padded)
   DBuf<float> cheb32;            // per node: Chebyshev points minus the node centre, FP32
   DBuf<double> node_ctr;         // per node: centre (midpoint of the Chebyshev interval)
+  std::vector<char> far_flag;    // host, per node: moments needed (far for some target)
+  // Set when another plan on the same source tree computes this plan's
+  // moments (its far nodes included) and is evaluated first on the same
+  // density: the far field reads that plan's far_idx, mom and mom32.
+  const TreePlan* mom_owner = nullptr;
   // reduction
   DBuf<int> slot_start;      // caller order, nt+1
   DBuf<double> slots;        // 3*nslot
@@ -282,7 +287,7 @@ __global__ __launch_bounds__(kBlock) void k_far(FarArgs A) {
           const float d1 = qy - t2[b];
           const float r2ab = d0 * d0 + d1 * d1;
           float v0 = 0.f, v1 = 0.f, v2 = 0.f;
-#pragma unroll 2
+#pragma unroll 4
           for (int c = 0; c < np1; c++, kk++) {
             const float* m = M + (size_t)kk * NC;
             const float d2 = qz - t3[c];
@@ -307,7 +312,7 @@ __global__ __launch_bounds__(kBlock) void k_far(FarArgs A) {
           const float gab = g0x * d0 + g0y * d1;
           const float d11 = d1 * d1, d01 = d0 * d1;
           float kab = 0.f, v2 = 0.f;
-#pragma unroll 2
+#pragma unroll 4
           for (int c = 0; c < np1; c++, kk++) {
             const float* m = M + (size_t)kk * NC;
             const float d2 = qz - t3[c];
@@ -452,6 +457,65 @@ static void bary_factors(int P, const double* t1, const double* t2, const double
   *D = 1.0 / (S[0] * S[1] * S[2]);
 }
 
+// Moment tables of the nodes flagged in far_flag: barycentric factors of
+// every member (host), member-chunk work items, per-item partial and final
+// moment buffers (device).  Sets far_idx, nfar_nodes and the plan's
+// moment_terms.
+static int build_moment_tables(hor_plan* P, const double* sx) {
+  hipStream_t s = P->ctx->stream;
+  TreePlan& TP = *P->tree;
+  const Tree& T = TP.T;
+  const bool dl = (P->kind == HOR_DL_ON || P->kind == HOR_DL_OFF);
+  const int nn = (int)T.nodes.size(), np1 = TP.np1, stride = 3 * np1 + 1;
+  std::vector<int> far_idx(nn + 1, -1), far_nodes;
+  for (int q = 0; q < nn; q++)
+    if (TP.far_flag[q]) { far_idx[q] = (int)far_nodes.size(); far_nodes.push_back(q); }
+  TP.nfar_nodes = (int)far_nodes.size();
+  std::vector<int> fac_off(TP.nfar_nodes + 1, 0);
+  for (int f = 0; f < TP.nfar_nodes; f++) {
+    const TreeNode& nd = T.nodes[far_nodes[f]];
+    fac_off[f + 1] = fac_off[f] + (nd.hi - nd.lo + 1) * stride;
+  }
+  std::vector<double> fac(std::max<size_t>(fac_off[TP.nfar_nodes], 1), 0.0);
+  double moment_terms = 0;
+  std::vector<int> mi_far, mi_m0, mi_m1, m_first(TP.nfar_nodes), m_nit(TP.nfar_nodes);
+  for (int f = 0; f < TP.nfar_nodes; f++) {
+    const int q = far_nodes[f], lo = T.nodes[q].lo, nm = T.nodes[q].hi - lo + 1;
+    m_first[f] = (int)mi_far.size();
+    for (int a = 0; a < nm; a += kMomChunk) {
+      mi_far.push_back(f);
+      mi_m0.push_back(a);
+      mi_m1.push_back(std::min(nm, a + kMomChunk));
+    }
+    m_nit[f] = (int)mi_far.size() - m_first[f];
+    double* F = &fac[fac_off[f]];
+    const double* cq = &T.cheb[3 * (size_t)np1 * q];
+    for (int m = 0; m < nm; m++) {
+      const int o = T.perm[lo + m];
+      double* fm = F + (size_t)m * stride;
+      bary_factors(np1 - 1, cq, cq + np1, cq + 2 * np1, sx[3 * o], sx[3 * o + 1], sx[3 * o + 2], fm, fm + np1,
+                   fm + 2 * np1, fm + 3 * np1);
+    }
+    moment_terms += (double)nm * TP.pflat;
+  }
+  P->moment_terms = moment_terms;
+  TP.n_mom_items = (int)mi_far.size();
+  HOR_TRY(TP.far_idx.upload(far_idx, s));
+  HOR_TRY(TP.far_node.upload(far_nodes.empty() ? std::vector<int>{0} : far_nodes, s));
+  HOR_TRY(TP.mom_item_far.upload(mi_far, s));
+  HOR_TRY(TP.mom_item_m0.upload(mi_m0, s));
+  HOR_TRY(TP.mom_item_m1.upload(mi_m1, s));
+  HOR_TRY(TP.mom_node_first.upload(m_first, s));
+  HOR_TRY(TP.mom_node_nitems.upload(m_nit, s));
+  HOR_TRY(TP.mom_part.alloc((size_t)std::max(TP.n_mom_items, 1) * TP.ncomp * TP.pflat));
+  HOR_TRY(TP.fac_off.upload(fac_off, s));
+  HOR_TRY(TP.fac.upload(fac, s));
+  HOR_TRY(TP.mom.alloc((size_t)std::max(TP.nfar_nodes, 1) * TP.ncomp * TP.pflat));
+  if (P->precision == HOR_MIXED)
+    HOR_TRY(TP.mom32.alloc((size_t)std::max(TP.nfar_nodes, 1) * (dl ? kNc32Dl : kNc32Sl) * TP.pflat));
+  return HOR_OK;
+}
+
 int tree_plan_create(hor_plan* P, const double* sx, const double* sn, const double* sa, const double* tx,
                      const double* tb, const double* tx0, const double* tn0, const double* ctr,
                      const hor_tree_params& tp) {
@@ -519,50 +583,18 @@ int tree_plan_create(hor_plan* P, const double* sx, const double* sn, const doub
   items(ftg, fsl, fi_node, fi_off, fi_cnt, f_tg, f_sl);
   TP.n_near_items = (int)ni_node.size();
   TP.n_far_items = (int)fi_node.size();
-  // far-field nodes: moments and barycentric factors
-  std::vector<int> far_idx(nn + 1, -1), fac_off;
-  std::vector<int> far_nodes;
-  for (int q = 0; q < nn; q++)
-    if (!ftg[q].empty()) { far_idx[q] = (int)far_nodes.size(); far_nodes.push_back(q); }
-  TP.nfar_nodes = (int)far_nodes.size();
-  fac_off.assign(TP.nfar_nodes + 1, 0);
-  for (int f = 0; f < TP.nfar_nodes; f++) {
-    int q = far_nodes[f];
-    fac_off[f + 1] = fac_off[f] + (hi[q] - lo[q] + 1) * (3 * TP.np1 + 1);
-  }
-  std::vector<double> fac(std::max<size_t>(fac_off[TP.nfar_nodes], 1), 0.0);
-  double near_pairs = 0, moment_terms = 0;
-  std::vector<int> mi_far, mi_m0, mi_m1, m_first(TP.nfar_nodes), m_nit(TP.nfar_nodes);
-  for (int f = 0; f < TP.nfar_nodes; f++) {
-    const int q = far_nodes[f], nm = hi[q] - lo[q] + 1, np1 = TP.np1, stride = 3 * np1 + 1;
-    m_first[f] = (int)mi_far.size();
-    for (int a = 0; a < nm; a += kMomChunk) {
-      mi_far.push_back(f);
-      mi_m0.push_back(a);
-      mi_m1.push_back(std::min(nm, a + kMomChunk));
-    }
-    m_nit[f] = (int)mi_far.size() - m_first[f];
-    double* F = &fac[fac_off[f]];
-    const double* cq = &T.cheb[3 * (size_t)np1 * q];
-    for (int m = 0; m < nm; m++) {
-      const int o = T.perm[lo[q] + m];
-      double* fm = F + (size_t)m * stride;
-      bary_factors(tp.P, cq, cq + np1, cq + 2 * np1, sx[3 * o], sx[3 * o + 1], sx[3 * o + 2], fm, fm + np1,
-                   fm + 2 * np1, fm + 3 * np1);
-    }
-    moment_terms += (double)nm * TP.pflat;
-  }
+  double near_pairs = 0;
   for (int q = 0; q < nn; q++) near_pairs += (double)ntg[q].size() * (hi[q] - lo[q] + 1);
   P->near_pairs = near_pairs;
   P->proxy_pairs = (double)f_tg.size() * TP.pflat;
-  P->moment_terms = moment_terms;
+  TP.far_flag.assign(nn, 0);
+  for (int q = 0; q < nn; q++) TP.far_flag[q] = !ftg[q].empty();
   P->direct_pairs = (double)P->ns * P->nt;
   // uploads
   HOR_TRY(TP.node_lo.upload(lo, s));
   HOR_TRY(TP.node_hi.upload(hi, s));
   HOR_TRY(TP.leaf_sub_off.upload(lso, s));
   HOR_TRY(TP.leaf_box.upload(lbox, s));
-  HOR_TRY(TP.far_idx.upload(far_idx, s));
   HOR_TRY(TP.cheb.upload(T.cheb, s));
   {
     // node centres (midpoints of the Chebyshev-Lobatto intervals) and the
@@ -590,24 +622,41 @@ int tree_plan_create(hor_plan* P, const double* sx, const double* sn, const doub
   HOR_TRY(TP.far_item_cnt.upload(fi_cnt, s));
   HOR_TRY(TP.far_tgt.upload(f_tg, s));
   HOR_TRY(TP.far_slot.upload(f_sl, s));
-  HOR_TRY(TP.far_node.upload(far_nodes.empty() ? std::vector<int>{0} : far_nodes, s));
-  TP.n_mom_items = (int)mi_far.size();
-  HOR_TRY(TP.mom_item_far.upload(mi_far, s));
-  HOR_TRY(TP.mom_item_m0.upload(mi_m0, s));
-  HOR_TRY(TP.mom_item_m1.upload(mi_m1, s));
-  HOR_TRY(TP.mom_node_first.upload(m_first, s));
-  HOR_TRY(TP.mom_node_nitems.upload(m_nit, s));
-  HOR_TRY(TP.mom_part.alloc((size_t)std::max(TP.n_mom_items, 1) * TP.ncomp * TP.pflat));
-  HOR_TRY(TP.fac_off.upload(fac_off, s));
-  HOR_TRY(TP.fac.upload(fac, s));
   HOR_TRY(TP.mq.alloc((size_t)P->nsp * 16));
-  HOR_TRY(TP.mom.alloc((size_t)std::max(TP.nfar_nodes, 1) * TP.ncomp * TP.pflat));
-  if (P->precision == HOR_MIXED)
-    HOR_TRY(TP.mom32.alloc((size_t)std::max(TP.nfar_nodes, 1) * (dl ? kNc32Dl : kNc32Sl) * TP.pflat));
   HOR_TRY(TP.slot_start.upload(TP.L.start, s));
   HOR_TRY(TP.slots.alloc(3 * (size_t)std::max(TP.nslot, 1)));
+  HOR_TRY(build_moment_tables(P, sx));
   // xi of the treecode off-surface double layer (set by setup_targets for
   // method == HOR_TREE) uses the given g0 of every target.
+  return HOR_OK;
+}
+
+// In the two-surface matvec the on-surface plan of a surface and its plan at
+// the other surface's points sum over the same source tree with the same
+// density, one after the other: the first (owner) computes the moments of
+// both plans' far nodes and the second (user) reads them.  The trees must be
+// identical (same sources, tree parameters and centre).
+int tree_plan_share_moments(hor_plan* owner, hor_plan* user, const double* sx) {
+  if (!owner || !user || !owner->tree || !user->tree)
+    return set_error(HOR_E_ARG, "moment sharing needs two treecode plans");
+  auto is_dl = [](int k) { return k == HOR_DL_ON || k == HOR_DL_OFF; };
+  TreePlan& O = *owner->tree;
+  TreePlan& U = *user->tree;
+  if (is_dl(owner->kind) != is_dl(user->kind) || owner->precision != user->precision ||
+      owner->ns != user->ns || O.np1 != U.np1 || O.T.nodes.size() != U.T.nodes.size() || O.T.perm != U.T.perm ||
+      O.T.cheb != U.T.cheb || U.mom_owner || O.mom_owner)
+    return set_error(HOR_E_ARG, "moment sharing: the plans' source trees differ");
+  for (size_t q = 0; q < O.T.nodes.size(); q++)
+    if (O.T.nodes[q].lo != U.T.nodes[q].lo || O.T.nodes[q].hi != U.T.nodes[q].hi)
+      return set_error(HOR_E_ARG, "moment sharing: the plans' source trees differ");
+  for (size_t q = 0; q < O.far_flag.size(); q++) O.far_flag[q] = O.far_flag[q] || U.far_flag[q];
+  HOR_TRY(build_moment_tables(owner, sx));
+  U.mom_owner = &O;
+  user->moment_terms = 0;
+  U.mom.release();
+  U.mom32.release();
+  U.mom_part.release();
+  U.fac.release();
   return HOR_OK;
 }
 
@@ -638,8 +687,8 @@ int tree_plan_eval(hor_plan* P, const double* dens_dev, const double* d0_dev, do
   else
     HOR_HIP_TRY(hipMemcpyAsync(P->tsub2.p, P->tsub.p, sizeof(double) * 3 * P->ntp, hipMemcpyDeviceToDevice, s));
   HOR_HIP_TRY(hipEventRecord(c->ev[2], s));
-  // 1. moments
-  if (TP.nfar_nodes > 0) {
+  // 1. moments (unless another plan computes them)
+  if (TP.nfar_nodes > 0 && !TP.mom_owner) {
     hipLaunchKernelGGL(k_moment_rhs, dim3((P->ns + kBlock - 1) / kBlock), dim3(kBlock), 0, s, P->ns, P->nsp,
                        dl ? 1 : 0, P->sn.p, P->sa.p, P->sd.p, TP.mq.p);
     MomArgs M;
@@ -676,9 +725,10 @@ int tree_plan_eval(hor_plan* P, const double* dens_dev, const double* d0_dev, do
     FarArgs F;
     F.np1 = TP.np1; F.pflat = TP.pflat; F.ntp = P->ntp; F.nslot = TP.nslot;
     F.item_node = TP.far_item_node.p; F.item_off = TP.far_item_off.p; F.item_cnt = TP.far_item_cnt.p;
-    F.tgt = TP.far_tgt.p; F.slot = TP.far_slot.p; F.far_idx = TP.far_idx.p;
-    F.cheb = TP.cheb.p; F.mom = TP.mom.p; F.tx = P->tx.p; F.tsub2 = P->tsub2.p; F.slots = TP.slots.p;
-    F.node_ctr = TP.node_ctr.p; F.cheb32 = TP.cheb32.p; F.mom32 = TP.mom32.p;
+    const TreePlan& MO = TP.mom_owner ? *TP.mom_owner : TP;
+    F.tgt = TP.far_tgt.p; F.slot = TP.far_slot.p; F.far_idx = MO.far_idx.p;
+    F.cheb = TP.cheb.p; F.mom = MO.mom.p; F.tx = P->tx.p; F.tsub2 = P->tsub2.p; F.slots = TP.slots.p;
+    F.node_ctr = TP.node_ctr.p; F.cheb32 = TP.cheb32.p; F.mom32 = MO.mom32.p;
     dim3 grid(TP.n_far_items);
     if (dl) {
       if (mixed) hipLaunchKernelGGL((k_far<true, true>), grid, dim3(kBlock), 0, s, F);

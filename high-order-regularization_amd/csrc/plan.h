@@ -59,6 +59,9 @@ int plan_eval_into(hor_plan* P, const double* dens_dev, const double* d0_dev, do
 int tree_plan_create(hor_plan* P, const double* sx, const double* sn, const double* sa,
                      const double* tx, const double* tb, const double* tx0, const double* tn0,
                      const double* ctr, const hor_tree_params& tp);
+// The user plan reads the moments the owner plan computes for both (same
+// source tree and density; the owner is evaluated first).
+int tree_plan_share_moments(hor_plan* owner, hor_plan* user, const double* sx);
 int tree_plan_eval(hor_plan* P, const double* dens_dev, const double* d0_dev, double* out_dev,
                    const double* base, double coef, int accumulate);
 

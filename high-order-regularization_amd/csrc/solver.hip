@@ -141,8 +141,9 @@ static int matvec_plans(hor_solver* S, hor_plan* const on[2], hor_plan* const of
   HOR_HIP_TRY(hipGetLastError());
   HOR_TRY(plan_eval_into(on[0], u1, nullptr, A1, u1, -factor, 0));           // u1 - 2/3 DL11
   HOR_TRY(plan_eval_into(off[0], u1, S->g0[1].p, A2, u2, -factor, 0));       // u2 - 2/3 DL12
-  HOR_TRY(plan_eval_into(off[1], u2, S->g0[0].p, A1, nullptr, -factor, 1));  // - 2/3 DL21
+  // on[k] before off[k]: a treecode off[k] reads the moments on[k] computes
   HOR_TRY(plan_eval_into(on[1], u2, nullptr, A2, nullptr, -factor, 1));      // - 2/3 DL22
+  HOR_TRY(plan_eval_into(off[1], u2, S->g0[0].p, A1, nullptr, -factor, 1));  // - 2/3 DL21
   return HOR_OK;
 }
 
@@ -164,6 +165,7 @@ static int ensure_fp64_plans(hor_solver* S) {
     HOR_TRY(hor_plan_create(c, HOR_DL_OFF, S->method, HOR_FP64, S->DEL, nk, S->x[k].data(), S->nrml[k].data(),
                             S->area[k].data(), no, S->x[o].data(), S->b[o].data(), S->x0[o].data(),
                             S->n0[o].data(), S->ctr[k], &S->tp[k], &S->off64[k]));
+    if (S->method == HOR_TREE) HOR_TRY(tree_plan_share_moments(S->on64[k], S->off64[k], S->x[k].data()));
   }
   return HOR_OK;
 }
@@ -223,6 +225,7 @@ extern "C" int hor_solver_create(hor_ctx* ctx, double h, double DEL, int method,
     HOR_TRY(hor_plan_create(ctx, HOR_DL_OFF, method, precision, DEL, me->n, me->x, me->nrml, me->area,
                             other->n, other->x, other->b, other->x0, other->n0, me->ctr, &S->tp[k],
                             &S->off[k]));
+    if (method == HOR_TREE) HOR_TRY(tree_plan_share_moments(S->on[k], S->off[k], me->x));
     // stencils of the points of surface k (their nearest points lie on the other surface)
     HOR_TRY(S->st[k].alloc(36 * (size_t)me->n));
     HOR_TRY(S->co[k].alloc(12 * (size_t)me->n));
