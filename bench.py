@@ -6,8 +6,10 @@ Workload: ellipsoid semi-axes (1, 0.8, 0.6) at the origin, grid h = 1/64 in
 partition of unity), delta = 4 (1/64)^(2/7) h^(5/7); targets = every grid node
 with |b| <= 0.3.  Densities: f = exact traction of the body translating with
 U = (1,0,0) (SL), g = seeded degree-3 spherical harmonics (DL).  One step =
-one FP64 direct SL_OFF + one DL_OFF evaluation over all band nodes with inputs
-resident in HBM.
+the FP64 direct SL_OFF and DL_OFF sums over all band nodes with inputs
+resident in HBM, as one fused pass (HOR_SLDL_OFF: both kernels for every pair,
+the two velocity fields of the separate operators); the separate SL_OFF +
+DL_OFF evaluations are timed beside it.
 
 Prints one JSON line (metric = pair interactions per second).
 """
@@ -391,50 +393,61 @@ def main():
     body, s, t, DEL = build_workload(h=args.h)
     ctx = hor.Context(0)
     t0 = time.time()
+    p_v = hor.Plan(ctx, hor.SLDL_OFF, s, t, DEL)
     p_sl = hor.Plan(ctx, hor.SL_OFF, s, t, DEL)
     p_dl = hor.Plan(ctx, hor.DL_OFF, s, t, DEL)
     log(f"plans built in {time.time() - t0:.1f}s")
     ns, nt = s["n"], t["n"]
     dev = {}
-    for name, arr in (("f", s["f"]), ("g", s["g"]), ("f0", t["f0"]), ("g0", t["g0"])):
+    for name, arr in (("f", s["f"]), ("g", s["g"]), ("f0", t["f0"]), ("g0", t["g0"]),
+                      ("fg", np.concatenate([s["f"], s["g"]])), ("fg0", np.concatenate([t["f0"], t["g0"]]))):
         dev[name] = ctx.alloc(arr.nbytes)
         ctx.upload(dev[name], np.ascontiguousarray(arr))
-    out_sl, out_dl = ctx.alloc(24 * nt), ctx.alloc(24 * nt)
+    out_sl, out_dl, out_v = ctx.alloc(24 * nt), ctx.alloc(24 * nt), ctx.alloc(48 * nt)
 
     def step():
-        p_sl.eval_dev(dev["f"], dev["f0"], out_sl)
-        k_sl = ctx.last_timing()[1]
-        p_dl.eval_dev(dev["g"], dev["g0"], out_dl)
-        k_dl = ctx.last_timing()[1]
-        return k_sl, k_dl
+        p_v.eval_dev(dev["fg"], dev["fg0"], out_v)
+        return ctx.last_timing()[1]
 
     for _ in range(args.warmup):
         step()
-    ksl, kdl = [], []
+    kv = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        a, b = step()
-        ksl.append(a)
-        kdl.append(b)
+        kv.append(step())
     wall = time.perf_counter() - t0
     ms_per_step = 1e3 * wall / args.steps
     pairs_per_step = 2.0 * ns * nt
     value = pairs_per_step / (wall / args.steps)
+    ref_v = ctx.download(out_v, np.zeros(6 * nt))
 
-    # the same step through the host-buffer entry points (densities in, velocities
+    # the same sums as two separate operators (SL_OFF, then DL_OFF)
+    ksl, kdl = [], []
+    p_sl.eval_dev(dev["f"], dev["f0"], out_sl)
+    t0 = time.perf_counter()
+    for _ in range(max(2, args.steps // 2)):
+        p_sl.eval_dev(dev["f"], dev["f0"], out_sl)
+        ksl.append(ctx.last_timing()[1])
+        p_dl.eval_dev(dev["g"], dev["g0"], out_dl)
+        kdl.append(ctx.last_timing()[1])
+    sep_ms = 1e3 * (time.perf_counter() - t0) / len(ksl)
+    ref_sl = ctx.download(out_sl, np.zeros(3 * nt))
+    ref_dl = ctx.download(out_dl, np.zeros(3 * nt))
+    fused_vs_separate = max(float(np.abs(ref_v[:3 * nt] - ref_sl).max() / np.abs(ref_sl).max()),
+                            float(np.abs(ref_v[3 * nt:] - ref_dl).max() / np.abs(ref_dl).max()))
+
+    # the same step through the host-buffer entry point (densities in, velocities
     # out over PCIe): reported beside the value, never as it
-    p_sl(s["f"], t["f0"])
+    fg, fg0 = np.concatenate([s["f"], s["g"]]), np.concatenate([t["f0"], t["g0"]])
+    p_v(fg, fg0)
     t0 = time.perf_counter()
     for _ in range(2):
-        p_sl(s["f"], t["f0"])
-        p_dl(s["g"], t["g0"])
+        p_v(fg, fg0)
     host_ms = 1e3 * (time.perf_counter() - t0) / 2
 
     # mixed-precision variant (FP32 far pairs, FP64 accumulation): time and error
     m_sl = hor.Plan(ctx, hor.SL_OFF, s, t, DEL, precision=hor.MIXED)
     m_dl = hor.Plan(ctx, hor.DL_OFF, s, t, DEL, precision=hor.MIXED)
-    ref_sl = ctx.download(out_sl, np.zeros(3 * nt))
-    ref_dl = ctx.download(out_dl, np.zeros(3 * nt))
     m_sl.eval_dev(dev["f"], dev["f0"], out_sl)
     m_dl.eval_dev(dev["g"], dev["g0"], out_dl)
     tm = time.perf_counter()
@@ -449,32 +462,36 @@ def main():
 
     # error against the exact translating-body velocity: u = U = (1,0,0) inside
     inside = t["b"] < 0
-    u_sl = ref_sl.reshape(-1, 3)
+    u_sl = ref_v[:3 * nt].reshape(-1, 3)
     err_inside = float(np.abs(u_sl[inside] - np.array([1.0, 0.0, 0.0])).max()) if inside.any() else None
 
-    # roofline of the dominant kernel (DL_OFF direct sum): algorithmic FP64
-    # flops per launch / average launch time (HIP events on the plan's stream)
+    # roofline of the dominant kernel (the fused SL + DL direct sum):
+    # algorithmic FP64 flops of both kernels per launch / average launch time
+    # (HIP events on the plan's stream)
     fnear = near_pair_fraction(s, t, DEL)
     dl_flops = ns * nt * (fnear * OPS["dl_reg"] + (1 - fnear) * OPS["dl_plain"])
     sl_flops = ns * nt * (fnear * OPS["sl_reg"] + (1 - fnear) * OPS["sl_plain"])
+    kv_ms = float(np.mean(kv))
     kdl_ms = float(np.mean(kdl))
     ksl_ms = float(np.mean(ksl))
-    achieved = dl_flops / (kdl_ms * 1e-3) / 1e12
+    achieved = (sl_flops + dl_flops) / (kv_ms * 1e-3) / 1e12
     roof = {"bound": "mfma", "compute": "fp64 vector ALU (peak equals the FP64 matrix peak)",
-            "kernel": "k_direct<DL_OFF,FP64>", "achieved": achieved, "peak": PEAK_FP64 / 1e12,
+            "kernel": "k_direct_sldl", "achieved": achieved, "peak": PEAK_FP64 / 1e12,
             "unit": "TFLOP/s", "frac": achieved * 1e12 / PEAK_FP64, "traffic": profile_traffic(),
-            "avg_kernel_ms": kdl_ms, "sl_kernel_ms": ksl_ms,
-            "sl_achieved_tflops": sl_flops / (ksl_ms * 1e-3) / 1e12,
-            "near_pair_fraction": fnear}
+            "avg_kernel_ms": kv_ms, "flop_per_launch": sl_flops + dl_flops,
+            "near_pair_fraction": fnear,
+            "separate": {"sl_kernel_ms": ksl_ms, "dl_kernel_ms": kdl_ms,
+                         "sl_achieved_tflops": sl_flops / (ksl_ms * 1e-3) / 1e12,
+                         "dl_achieved_tflops": dl_flops / (kdl_ms * 1e-3) / 1e12}}
     counted = profile_valu()
-    if counted and "k_direct<DL_OFF,FP64>" in counted:
-        c = counted["k_direct<DL_OFF,FP64>"]
-        t_dl = kdl_ms * 1e-3
+    if counted and "k_direct_sldl" in counted:
+        c = counted["k_direct_sldl"]
+        t_v = kv_ms * 1e-3
         roof["counted"] = {
             "fp64_flop_per_launch": c["fp64_flop_per_launch"],
-            "fp64_tflops": c["fp64_flop_per_launch"] / t_dl / 1e12,
-            "fp64_frac": c["fp64_flop_per_launch"] / t_dl / PEAK_FP64,
-            "valu_issue_frac": c["counters"]["SQ_INSTS_VALU"] * 64 / t_dl / VALU_ISSUE,
+            "fp64_tflops": c["fp64_flop_per_launch"] / t_v / 1e12,
+            "fp64_frac": c["fp64_flop_per_launch"] / t_v / PEAK_FP64,
+            "valu_issue_frac": c["counters"]["SQ_INSTS_VALU"] * 64 / t_v / VALU_ISSUE,
             "note": "hardware-counted FP64 FLOPs (rocprofv3 VALU counters, profiles/direct_band_h64_valu_pmc.json) "
                     "over this run's launch time; the FLOPs the kernel executes, including the 1/r refinement"}
 
@@ -537,10 +554,13 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": "extend-to-grid band: ellipsoid (1,0.8,0.6), h=1/64, box [-1.5,1.5]^3, "
-                               "|b|<=0.3, FP64 direct SL_OFF + DL_OFF",
+                               "|b|<=0.3, FP64 direct SL_OFF + DL_OFF (one fused pass)",
                    "sources": ns, "targets": nt, "delta": DEL, "pairs_per_step": pairs_per_step},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "separate_ops": {"ms_per_step": sep_ms, "value": pairs_per_step / (sep_ms * 1e-3),
+                         "max_rel_diff_vs_fused": fused_vs_separate,
+                         "note": "the same step as an SL_OFF and a DL_OFF evaluation"},
         "host_buffers": {"ms_per_step": host_ms, "value": pairs_per_step / (host_ms * 1e-3),
                          "note": "hor_plan_eval with host arrays: density upload, sums, velocity download"},
         "mixed": {"ms_per_step": mixed_ms, "value": pairs_per_step / (mixed_ms * 1e-3),
@@ -554,6 +574,7 @@ def main():
         ctx.free(p)
     ctx.free(out_sl)
     ctx.free(out_dl)
+    ctx.free(out_v)
     print(json.dumps(line), flush=True)
 
 

@@ -151,6 +151,13 @@ struct __align__(16) SrcRec {
   double d0, d1, d2, pad1;
 };
 
+// Record of the fused Stokeslet + stresslet sum: position, a n, a f, g.
+struct __align__(16) SrcRecV {
+  double x, y, z, nx;
+  double ny, nz, f0, f1;
+  double f2, g0, g1, g2;
+};
+
 struct __align__(16) SrcRec32 {
   float x, y, z, a;
   float nx, ny, nz, pad0;

@@ -21,18 +21,20 @@ struct DirectArgs {
   const double* sa;                // nsp weights
   const double* sd;                // 3*nsp density (f or g)
   const struct SrcRec* rec;        // nsp source records (sorted), this evaluation's density
+  const struct SrcRecV* recv;      // SLDL_OFF: nsp fused records
   const struct SrcRec32* rec32;    // FP32 copies (mixed precision) or null; direct plans store
                                    // y - (centre of the source's 64-subtile box)
   const double* sub_box;           // 6 per 64-source subtile (lo xyz, hi xyz)
   const double* tx;                // 3*ntp positions
   const double* tcoef;             // 12*ntp regularization polynomial coefficients
-  const double* tsub;              // SL: ntp f0.n0; DL: 3*ntp g0; P: ntp f0.n0, 3*ntp n0 x f0
+  const double* tsub;              // SL: ntp f0.n0; DL: 3*ntp g0; P: ntp f0.n0, 3*ntp n0 x f0;
+                                   // SLDL: ntp f0.n0, 3*ntp g0
   const double* tx0;               // DL_OFF: 3*ntp nearest point
   const double* tn0;               // DL_OFF: 3*ntp nearest normal
   const double* tb;                // DL_OFF: ntp signed distance
   const int* tnear;                // ntp: 1 = regularized target (|b|/delta <= 8 or on surface)
   const double* wave_box;          // 8 per 64-target group: lo xyz, hi xyz, all-far flag, pad
-  double* partial;                 // nchunk*3*ntp partial sums
+  double* partial;                 // nchunk*3*ntp partial sums (nchunk*6*ntp for SLDL)
   const double* dlon_tab;          // kDlOnTable: the on-surface stresslet kernel (staged in LDS)
   const double* ae_tab;            // kErfExpTable: erf(rho)/rho and exp(-rho^2) (staged in LDS)
 };
@@ -47,6 +49,10 @@ int launch_pressure_finalize(const double* partial, int nchunk, int nt, int ntp,
 int launch_gather3(const double* dens, const int* perm, int n, int np, const double* sx, const double* sn,
                    const double* sa, double* sd, struct SrcRec* rec, struct SrcRec32* rec32,
                    const double* sub_box, int scale_density, hipStream_t s);
+int launch_sldl_finalize(const double* partial, int nchunk, int nt, int ntp, double scale_sl, double scale_dl,
+                         const double* xi, const double* g0, const int* tperm, double* out, hipStream_t s);
+int launch_gather_sldl(const double* dens, const int* perm, int n, int np, const double* sx, const double* sn,
+                       const double* sa, struct SrcRecV* recv, hipStream_t s);
 int launch_target_sub(int mode, int nt, int ntp, const int* tperm, const int* tnear,
                       const double* d0, const double* n0_orig, const double* dens,
                       const double* sn_orig, double* out, hipStream_t s);

@@ -46,6 +46,49 @@ __global__ __launch_bounds__(kBlock) void k_direct(DirectArgs A) {
   }
 }
 
+// Stokeslet + stresslet at off-surface targets in one pass (HOR_SLDL_OFF):
+// the partial sums of both kernels, 6 per target per chunk.
+__global__ __launch_bounds__(kBlock, 4) void k_direct_sldl(DirectArgs A) {
+  const double* tab = stage_table<HOR_SL_OFF>(A);
+  const int tid = threadIdx.x;
+  const int i = blockIdx.x * kBlock + tid;     // sorted target (ntp is a multiple of kBlock)
+  const int wave = i / kWave;
+  Tgt t;
+  load_target<HOR_SLDL_OFF>(A, i, t);
+  const double* wb = A.wave_box + 8 * wave;
+  const bool all_far = wb[6] != 0.0;
+  double us[3] = {0.0, 0.0, 0.0}, ud[3] = {0.0, 0.0, 0.0};
+  const int sub0 = blockIdx.y * A.subs_per_chunk;
+  const int sub1 = min(A.nsub, sub0 + A.subs_per_chunk);
+  range_sum_sldl(A, sub0 * kWave, min(A.ns, sub1 * kWave), sub0, wb, all_far, t, tab, us, ud);
+  double* P = A.partial + (size_t)blockIdx.y * 6 * A.ntp;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    P[(size_t)k * A.ntp + i] = us[k];
+    P[(size_t)(3 + k) * A.ntp + i] = ud[k];
+  }
+}
+
+// The two results of the fused sum in the caller's order: out[0, 3nt) =
+// scale_sl*sum (the SL_OFF finalize), out[3nt, 6nt) = scale_dl*sum + xi*g0
+// (the DL_OFF finalize), chunk partials added in chunk order.
+__global__ void k_sldl_finalize(const double* partial, int nchunk, int nt, int ntp, double scale_sl,
+                                double scale_dl, const double* xi, const double* g0, const int* tperm,
+                                double* out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nt) return;
+  int o = tperm[i];
+  for (int k = 0; k < 3; k++) {
+    double s = 0.0, d = 0.0;
+    for (int c = 0; c < nchunk; c++) {
+      s += partial[((size_t)c * 6 + k) * ntp + i];
+      d += partial[((size_t)c * 6 + 3 + k) * ntp + i];
+    }
+    out[3 * o + k] = s * scale_sl;
+    out[3 * (size_t)nt + 3 * o + k] = d * scale_dl + xi[i] * g0[(size_t)k * ntp + i];
+  }
+}
+
 // Pressure: one value per target, v = scale*sum + xi*(f0.n0) with xi = -1
 // inside (b < -1e-12, stokes.cpp:198-200), combined into out like
 // k_direct_finalize.
@@ -124,6 +167,23 @@ __global__ void k_gather3(const double* dens, const int* perm, int n, int np, co
   }
 }
 
+// Fused records (HOR_SLDL_OFF), dens = [f | g]: recv[j] <- (y, a n, a f, g)
+// of sorted source j, the SL and DL records' values.
+__global__ void k_gather_sldl(const double* dens, const int* perm, int n, int np, const double* sx,
+                              const double* sn, const double* sa, SrcRecV* recv) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  int o = perm[j];
+  const double a = sa[j];
+  const double* g = dens + 3 * (size_t)n;
+  SrcRecV r;
+  r.x = sx[j]; r.y = sx[np + j]; r.z = sx[2 * np + j];
+  r.nx = a * sn[j]; r.ny = a * sn[np + j]; r.nz = a * sn[2 * np + j];
+  r.f0 = dens[3 * o] * a; r.f1 = dens[3 * o + 1] * a; r.f2 = dens[3 * o + 2] * a;
+  r.g0 = g[3 * o]; r.g1 = g[3 * o + 1]; r.g2 = g[3 * o + 2];
+  recv[j] = r;
+}
+
 // Per-target subtraction data (sorted order):
 //   mode 0 (SL_OFF): sub = near ? d0.n0 : 0       (stokeslets.cpp:72)
 //   mode 1 (SL_ON):  sub = f_o . n_o of source o  (stokeslets.cpp:167)
@@ -131,6 +191,7 @@ __global__ void k_gather3(const double* dens, const int* perm, int n, int np, co
 //   mode 3 (DL_ON):  g0 = g_o of source o          (stresslets.cpp:212-214)
 //   mode 4 (DL_OFF): g0 = d0 for every target      (treecode proxies and jump)
 //   mode 5 (P_OFF):  [d0.n0, n0 x d0] for every target (stokes.cpp:146-147)
+//   mode 6 (SLDL_OFF, d0 = [f0 | g0]): [near ? f0.n0 : 0, near ? g0 : 0]
 __global__ void k_target_sub(int mode, int nt, int ntp, const int* tperm, const int* tnear,
                              const double* d0, const double* n0_orig, const double* dens,
                              const double* sn_orig, double* out) {
@@ -139,7 +200,7 @@ __global__ void k_target_sub(int mode, int nt, int ntp, const int* tperm, const 
   if (i >= nt) {
     out[i] = 0;
     if (mode >= 2) { out[ntp + i] = 0; out[2 * ntp + i] = 0; }
-    if (mode == 5) out[3 * ntp + i] = 0;
+    if (mode == 5 || mode == 6) out[3 * ntp + i] = 0;
     return;
   }
   int o = tperm[i];
@@ -155,6 +216,11 @@ __global__ void k_target_sub(int mode, int nt, int ntp, const int* tperm, const 
     for (int k = 0; k < 3; k++) out[(size_t)k * ntp + i] = nearp ? d0[3 * o + k] : 0.0;
   } else if (mode == 3) {
     for (int k = 0; k < 3; k++) out[(size_t)k * ntp + i] = dens[3 * o + k];
+  } else if (mode == 6) {
+    out[i] = nearp ? d0[3 * o] * n0_orig[3 * o] + d0[3 * o + 1] * n0_orig[3 * o + 1] +
+                         d0[3 * o + 2] * n0_orig[3 * o + 2]
+                   : 0.0;
+    for (int k = 0; k < 3; k++) out[(size_t)(1 + k) * ntp + i] = nearp ? d0[3 * (size_t)nt + 3 * o + k] : 0.0;
   } else if (mode == 5) {
     const double* f = d0 + 3 * o;
     const double* n = n0_orig + 3 * o;
@@ -182,6 +248,12 @@ __global__ void k_target_f0n0_all(int nt, int ntp, const int* tperm, const doubl
 
 int launch_direct(const DirectArgs& A, int kind, bool mixed, int nchunk, hipStream_t s) {
   dim3 grid(A.ntp / kBlock, nchunk), block(kBlock);
+  if (kind == HOR_SLDL_OFF) {
+    if (mixed) return set_error(HOR_E_ARG, "the fused Stokeslet + stresslet sum is FP64 only");
+    hipLaunchKernelGGL(k_direct_sldl, grid, block, lds_bytes(HOR_SL_OFF), s, A);
+    HOR_HIP_TRY(hipGetLastError());
+    return HOR_OK;
+  }
 #define HOR_LAUNCH(K, M) hipLaunchKernelGGL((k_direct<K, M>), grid, block, lds_bytes(K), s, A)
   if (!mixed) {
     switch (kind) {
@@ -211,6 +283,23 @@ int launch_finalize(const double* partial, int nchunk, int nt, int ntp, double s
   if (nt == 0) return HOR_OK;
   hipLaunchKernelGGL(k_direct_finalize, dim3((nt + 255) / 256), dim3(256), 0, s, partial, nchunk, nt,
                      ntp, scale, xi, g0, tperm, out, base, coef, accumulate);
+  HOR_HIP_TRY(hipGetLastError());
+  return HOR_OK;
+}
+
+int launch_sldl_finalize(const double* partial, int nchunk, int nt, int ntp, double scale_sl, double scale_dl,
+                         const double* xi, const double* g0, const int* tperm, double* out, hipStream_t s) {
+  if (nt == 0) return HOR_OK;
+  hipLaunchKernelGGL(k_sldl_finalize, dim3((nt + 255) / 256), dim3(256), 0, s, partial, nchunk, nt, ntp, scale_sl,
+                     scale_dl, xi, g0, tperm, out);
+  HOR_HIP_TRY(hipGetLastError());
+  return HOR_OK;
+}
+
+int launch_gather_sldl(const double* dens, const int* perm, int n, int np, const double* sx, const double* sn,
+                       const double* sa, SrcRecV* recv, hipStream_t s) {
+  if (n == 0) return HOR_OK;
+  hipLaunchKernelGGL(k_gather_sldl, dim3((n + 255) / 256), dim3(256), 0, s, dens, perm, n, np, sx, sn, sa, recv);
   HOR_HIP_TRY(hipGetLastError());
   return HOR_OK;
 }

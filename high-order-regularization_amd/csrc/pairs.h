@@ -231,6 +231,100 @@ __device__ __forceinline__ void dl_off_full(const SrcRec& s, const Tgt& t, doubl
   u[2] += c2;
 }
 
+// ---- Stokeslet + stresslet at one off-surface target (HOR_SLDL_OFF): the
+// pair's separation, r^2 and 1/r are computed once; the single-layer terms
+// are sl_plain's / sl_full's and the double-layer terms dl_plain's /
+// dl_off_full's, expression for expression.
+__device__ __forceinline__ void sldl_plain(const SrcRecV& s, const Tgt& t, double us[3], double ud[3]) {
+  double dx = t.px - s.x, dy = t.py - s.y, dz = t.pz - s.z;
+  double r2 = dx * dx + dy * dy + dz * dz;
+  double ri = rsqrt64(r2);
+  double fx = s.f0 - t.sub * s.nx, fy = s.f1 - t.sub * s.ny, fz = s.f2 - t.sub * s.nz;
+  double tq = (fx * dx + fy * dy + fz * dz) * (ri * ri);
+  us[0] += (fx + dx * tq) * ri;
+  us[1] += (fy + dy * tq) * ri;
+  us[2] += (fz + dz * tq) * ri;
+  double ri2 = ri * ri;
+  double gdx = (s.g0 - t.g0x) * dx + (s.g1 - t.g0y) * dy + (s.g2 - t.g0z) * dz;
+  double ndx = s.nx * dx + s.ny * dy + s.nz * dz;
+  double k = gdx * ndx * (ri2 * ri2 * ri);
+  ud[0] += dx * k;
+  ud[1] += dy * k;
+  ud[2] += dz * k;
+}
+
+__device__ __forceinline__ void sldl_full(const SrcRecV& s, const Tgt& t, double inv_del, double inv_del2,
+                                          double self1, double self2, const double* __restrict__ tab, double us[3],
+                                          double ud[3]) {
+  double dx = t.px - s.x, dy = t.py - s.y, dz = t.pz - s.z;
+  double r2 = dx * dx + dy * dy + dz * dz;
+  double fx = s.f0 - t.sub * s.nx, fy = s.f1 - t.sub * s.ny, fz = s.f2 - t.sub * s.nz;
+  double fdx = fx * dx + fy * dy + fz * dz;
+  double gx = s.g0 - t.g0x, gy = s.g1 - t.g0y, gz = s.g2 - t.g0z;
+  double s1, s2, H2 = 0.0, H3;
+  bool plain = false;
+  if (t.nearp && r2 < 1e-28) {
+    s1 = self1;
+    s2 = self2;
+    H2 = self2;
+    H3 = 0.0;
+  } else {
+    const double ri = rsqrt64(r2);
+    const double ri2 = ri * ri;
+    const double sq = r2 * inv_del2;   // rho^2
+    if (!t.nearp || sq > 64.0) {
+      s1 = ri;
+      s2 = ri * ri * ri;
+      plain = true;
+      H3 = ri2 * ri2 * ri;
+    } else {
+      double A, E;
+      erf_exp_AE(tab, sq, A, E);
+      const double* q = t.q;
+      const double p1 = q[0] + sq * (q[1] + sq * q[2]);
+      const double p2 = q[3] + sq * (q[4] + sq * (q[5] + sq * q[6]));
+      const double p3 = q[7] + sq * (q[8] + sq * (q[9] + sq * (q[10] + sq * q[11])));
+      s1 = (A + p1 * E) * inv_del;
+      s2 = (A + p2 * E) * inv_del * (ri * ri);
+      const double w2 = inv_del * ri2;
+      H2 = (A + p2 * E) * w2;
+      H3 = (A + p3 * E) * (w2 * ri2);
+    }
+  }
+  double HS = s2 * fdx;
+  us[0] += fx * s1 + dx * HS;
+  us[1] += fy * s1 + dy * HS;
+  us[2] += fz * s1 + dz * HS;
+  double c0, c1, c2;
+  if (plain) {
+    double gdx = gx * dx + gy * dy + gz * dz;
+    double ndx = s.nx * dx + s.ny * dy + s.nz * dz;
+    double k = gdx * ndx * H3;
+    c0 = dx * k;
+    c1 = dy * k;
+    c2 = dz * k;
+  } else {
+    double xh0 = s.x - t.x0x, xh1 = s.y - t.x0y, xh2 = s.z - t.x0z;
+    double sig2 = (xh0 * xh0 + xh1 * xh1 + xh2 * xh2) - 2.0 * t.b * (xh0 * t.n0x + xh1 * t.n0y + xh2 * t.n0z);
+    double g_n0 = gx * t.n0x + gy * t.n0y + gz * t.n0z;
+    double g_xh = gx * xh0 + gy * xh1 + gz * xh2;
+    double n0_n = t.n0x * s.nx + t.n0y * s.ny + t.n0z * s.nz;
+    double xh_n = xh0 * s.nx + xh1 * s.ny + xh2 * s.nz;
+    double m1a = g_n0 * n0_n;
+    double m1b = g_xh * n0_n + g_n0 * xh_n;
+    double m2a = g_xh * xh_n;
+    double m2b = t.b * m1b;
+    double A1 = H2 - sig2 * H3;
+    double Bd = m1a * A1 + m2a * H3, Bn = m1b * A1, Bx = m2b * H3;
+    c0 = dx * Bd - t.n0x * Bn + xh0 * Bx;
+    c1 = dy * Bd - t.n0y * Bn + xh1 * Bx;
+    c2 = dz * Bd - t.n0z * Bn + xh2 * Bx;
+  }
+  ud[0] += c0;
+  ud[1] += c1;
+  ud[2] += c2;
+}
+
 // ---- DL on-surface pair, full branch structure (stresslets.cpp:244-269).
 // On-surface stresslet pair (Stokes_DL_7ord_onSurf, utilities.cpp:291-323):
 // H3 = F(rho)/r^5 with F = erf(rho) + p3(rho^2) rho e^{-rho^2}; within 8 delta
@@ -443,9 +537,9 @@ __device__ __forceinline__ void load_target(const DirectArgs& A, int i, Tgt& t) 
   t.x0x = t.x0y = t.x0z = t.n0x = t.n0y = t.n0z = t.b = 0;
   if (KIND == HOR_SL_ON || KIND == HOR_SL_OFF) {
     t.sub = A.tsub[i];
-  } else if (KIND == HOR_P_OFF) {
+  } else if (KIND == HOR_P_OFF || KIND == HOR_SLDL_OFF) {
     t.sub = A.tsub[i];             // f0.n0
-    t.g0x = A.tsub[ntp + i];       // n0 x f0
+    t.g0x = A.tsub[ntp + i];       // P: n0 x f0; SLDL: g0
     t.g0y = A.tsub[2 * ntp + i];
     t.g0z = A.tsub[3 * ntp + i];
   } else {
@@ -453,7 +547,7 @@ __device__ __forceinline__ void load_target(const DirectArgs& A, int i, Tgt& t) 
     t.g0y = A.tsub[ntp + i];
     t.g0z = A.tsub[2 * ntp + i];
   }
-  if (KIND == HOR_DL_OFF) {
+  if (KIND == HOR_DL_OFF || KIND == HOR_SLDL_OFF) {
     t.x0x = A.tx0[i]; t.x0y = A.tx0[ntp + i]; t.x0z = A.tx0[2 * ntp + i];
     t.n0x = A.tn0[i]; t.n0y = A.tn0[ntp + i]; t.n0z = A.tn0[2 * ntp + i];
     t.b = A.tb[i];
@@ -551,6 +645,37 @@ __device__ __forceinline__ void range_sum(const DirectArgs& A, int j0, int j1, i
         u[0] += (double)w[0];
         u[1] += (double)w[1];
         u[2] += (double)w[2];
+      }
+    }
+  }
+}
+
+// Stokeslet + stresslet over the sorted sources [j0, j1) (HOR_SLDL_OFF, FP64):
+// the subtile test, the per-source vote and the pair functions of the
+// separate SL_OFF and DL_OFF sums, in the same order.
+__device__ __forceinline__ void range_sum_sldl(const DirectArgs& A, int j0, int j1, int sub0, const double* wb,
+                                               bool all_far, const Tgt& t, const double* __restrict__ tab,
+                                               double us[3], double ud[3]) {
+  const double rp = 1.7724538509055160273;
+  const double inv_del = 1.0 / A.DEL;
+  const double inv_del2 = inv_del * inv_del;
+  const double self1 = 32.0 / 5.0 / rp / A.DEL;
+  const double self2 = 256.0 / 15.0 / rp / (A.DEL * A.DEL * A.DEL);
+  const SrcRecV* __restrict__ R = A.recv;
+  for (int base = j0, sb = sub0; base < j1; base += kWave, sb++) {
+    const int end = min(j1, base + kWave);
+    const double* box = A.sub_box + 6 * sb;
+    if (all_far || boxes_far(wb, box, A.far_d2)) {
+#pragma unroll 2
+      for (int j = base; j < end; j++) sldl_plain(R[j], t, us, ud);
+    } else {
+      for (int j = base; j < end; j++) {
+        const SrcRecV s = R[j];
+        const double dx = t.px - s.x, dy = t.py - s.y, dz = t.pz - s.z;
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        const bool need = t.nearp && !(r2 > A.near_r2);
+        if (__any(need ? 1 : 0)) sldl_full(s, t, inv_del, inv_del2, self1, self2, tab, us, ud);
+        else sldl_plain(s, t, us, ud);
       }
     }
   }

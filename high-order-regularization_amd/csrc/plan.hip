@@ -114,7 +114,8 @@ bool target_is_near(const hor_plan* P, double b) {
   int kind = P->kind;
   double DEL = P->DEL;
   if (kind == HOR_SL_ON || kind == HOR_DL_ON || P->all_near) return true;
-  if ((kind == HOR_SL_OFF || kind == HOR_P_OFF) && P->method == HOR_DIRECT) return !(std::fabs(b / DEL) > 8.0);
+  if ((kind == HOR_SL_OFF || kind == HOR_P_OFF || kind == HOR_SLDL_OFF) && P->method == HOR_DIRECT)
+    return !(std::fabs(b / DEL) > 8.0);  // the same class as the DL_OFF rule below
   return !(std::fabs(b) / DEL > 8.0);  // stresslets.cpp:41, KITC.cpp:856, 970
 }
 
@@ -149,7 +150,7 @@ int setup_targets(hor_plan* P, const double* tx, const double* tb, const double*
   P->ntp = ntp;
   std::vector<double> X(3 * (size_t)ntp), C((size_t)kNumCoef * ntp, 0.0), X0, N0, Bv, XI;
   std::vector<int> NEAR(ntp);
-  bool off_dl = (kind == HOR_DL_OFF);
+  bool off_dl = (kind == HOR_DL_OFF || kind == HOR_SLDL_OFF);
   if (off_dl) {
     X0.assign(3 * (size_t)ntp, 0.0);
     N0.assign(3 * (size_t)ntp, 0.0);
@@ -175,7 +176,7 @@ int setup_targets(hor_plan* P, const double* tx, const double* tb, const double*
     if (i >= nt) continue;
     // jump coefficient of the double layer
     if (kind == HOR_DL_ON) XI[i] = 0.5;                                   // stresslets.cpp:276-278
-    if (kind == HOR_DL_OFF) {
+    if (kind == HOR_DL_OFF || kind == HOR_SLDL_OFF) {
       if (P->method == HOR_DIRECT)
         XI[i] = (nearp && (b < 0.0 || std::fabs(b) < 1.0e-14)) ? 1.0 : 0.0;  // stresslets.cpp:174
       else
@@ -188,7 +189,7 @@ int setup_targets(hor_plan* P, const double* tx, const double* tb, const double*
   std::vector<double> WB(8 * (size_t)nw);
   for (int w = 0; w < nw; w++) {
     double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-    bool all_far = (kind == HOR_SL_OFF || kind == HOR_DL_OFF || kind == HOR_P_OFF);
+    bool all_far = (kind == HOR_SL_OFF || kind == HOR_DL_OFF || kind == HOR_P_OFF || kind == HOR_SLDL_OFF);
     for (int i = w * kWave; i < (w + 1) * kWave; i++) {
       for (int k = 0; k < 3; k++) {
         double v = X[(size_t)k * ntp + i];
@@ -212,12 +213,13 @@ int setup_targets(hor_plan* P, const double* tx, const double* tb, const double*
     HOR_TRY(P->tn0.upload(N0, s));
     HOR_TRY(P->tb.upload(Bv, s));
   }
-  if (kind == HOR_SL_OFF || kind == HOR_DL_OFF || kind == HOR_P_OFF)
+  if (kind == HOR_SL_OFF || kind == HOR_DL_OFF || kind == HOR_P_OFF || kind == HOR_SLDL_OFF)
     HOR_TRY(P->n0_orig.upload(tn0, 3 * (size_t)nt, s));
-  HOR_TRY(P->tsub.alloc((kind == HOR_P_OFF ? 4 : 3) * (size_t)ntp));
+  const size_t per = kind == HOR_SLDL_OFF ? 6 : 3;   // d0 and result values per target
+  HOR_TRY(P->tsub.alloc((kind == HOR_P_OFF || kind == HOR_SLDL_OFF ? 4 : 3) * (size_t)ntp));
   HOR_TRY(P->tsub2.alloc(3 * (size_t)ntp));
-  HOR_TRY(P->in_d0.alloc(3 * (size_t)std::max(nt, 1)));
-  HOR_TRY(P->out_buf.alloc(3 * (size_t)std::max(nt, 1)));
+  HOR_TRY(P->in_d0.alloc(per * (size_t)std::max(nt, 1)));
+  HOR_TRY(P->out_buf.alloc(per * (size_t)std::max(nt, 1)));
   (void)sx_orig;
   return HOR_OK;
 }
@@ -258,10 +260,11 @@ int setup_sources(hor_plan* P, const std::vector<int>& perm, const double* sx, c
   HOR_TRY(P->sa.upload(A, s));
   HOR_TRY(P->sub_box.upload(SB, s));
   HOR_TRY(P->sd.alloc(3 * (size_t)nsp));
-  HOR_TRY(P->rec.alloc(nsp));
+  if (P->kind == HOR_SLDL_OFF) HOR_TRY(P->recv.alloc(nsp));
+  else HOR_TRY(P->rec.alloc(nsp));
   if (P->precision == HOR_MIXED) HOR_TRY(P->rec32.alloc(nsp));
   HOR_TRY(P->sn_orig.upload(sn, 3 * (size_t)ns, s));
-  HOR_TRY(P->in_dens.alloc(3 * (size_t)ns));
+  HOR_TRY(P->in_dens.alloc((P->kind == HOR_SLDL_OFF ? 6 : 3) * (size_t)ns));
   return HOR_OK;
 }
 
@@ -277,7 +280,7 @@ DirectArgs direct_args(hor_plan* P) {
   A.near_r2 = 64.0 * P->DEL * P->DEL * (1.0 + 1e-12);
   A.near64_r2 = 16.0 * P->DEL * P->DEL;
   A.sx = P->sx.p; A.sn = P->sn.p; A.sa = P->sa.p; A.sd = P->sd.p; A.sub_box = P->sub_box.p;
-  A.rec = P->rec.p; A.rec32 = P->precision == HOR_MIXED ? P->rec32.p : nullptr;
+  A.rec = P->rec.p; A.recv = P->recv.p; A.rec32 = P->precision == HOR_MIXED ? P->rec32.p : nullptr;
   A.tx = P->tx.p; A.tcoef = P->tcoef.p; A.tsub = P->tsub.p;
   A.tx0 = P->tx0.p; A.tn0 = P->tn0.p; A.tb = P->tb.p;
   A.tnear = P->tnear.p; A.wave_box = P->wave_box.p; A.partial = P->partial.p;
@@ -297,6 +300,7 @@ int prepare_targets(hor_plan* P, const double* dens_dev, const double* d0_dev) {
   hipStream_t s = P->ctx->stream;
   int mode = (P->kind == HOR_SL_OFF) ? 0 : (P->kind == HOR_SL_ON) ? 1 : (P->kind == HOR_DL_OFF) ? 2 : 3;
   if (P->kind == HOR_P_OFF) mode = 5;
+  if (P->kind == HOR_SLDL_OFF) mode = 6;
   if (P->on_at) mode = 4;  // on-surface stresslet at given targets: g0 = d0
   HOR_TRY(launch_target_sub(mode, P->nt, P->ntp, P->tperm.p, P->tnear.p, d0_dev, P->n0_orig.p, dens_dev,
                             P->sn_orig.p, P->tsub.p, s));
@@ -316,7 +320,7 @@ static int plan_create_direct(hor_plan* P, const double* sx, const double* sn, c
   int nchunk = std::max(1, std::min(P->nsub, want));
   P->subs_per_chunk = (P->nsub + nchunk - 1) / nchunk;
   P->nchunk = (P->nsub + P->subs_per_chunk - 1) / P->subs_per_chunk;
-  HOR_TRY(P->partial.alloc((size_t)P->nchunk * 3 * P->ntp));
+  HOR_TRY(P->partial.alloc((size_t)P->nchunk * (P->kind == HOR_SLDL_OFF ? 6 : 3) * P->ntp));
   P->direct_pairs = (double)P->ns * (double)P->nt;
   return HOR_OK;
 }
@@ -325,6 +329,18 @@ static int plan_eval_direct(hor_plan* P, const double* dens_dev, const double* d
                             const double* base, double coef, int accumulate) {
   hor_ctx* c = P->ctx;
   hipStream_t s = c->stream;
+  if (P->kind == HOR_SLDL_OFF) {
+    if (base || coef != 1.0 || accumulate)
+      return set_error(HOR_E_ARG, "the fused Stokeslet + stresslet sum writes its result only");
+    HOR_TRY(launch_gather_sldl(dens_dev, P->sperm.p, P->ns, P->nsp, P->sx.p, P->sn.p, P->sa.p, P->recv.p, s));
+    HOR_TRY(prepare_targets(P, dens_dev, d0_dev));
+    DirectArgs A = direct_args(P);
+    HOR_HIP_TRY(hipEventRecord(c->ev[2], s));
+    HOR_TRY(launch_direct(A, P->kind, false, P->nchunk, s));
+    HOR_HIP_TRY(hipEventRecord(c->ev[3], s));
+    return launch_sldl_finalize(P->partial.p, P->nchunk, P->nt, P->ntp, op_scale(HOR_SL_OFF), op_scale(HOR_DL_OFF),
+                                P->xi.p, P->tsub.p + P->ntp, P->tperm.p, out_dev, s);
+  }
   HOR_TRY(launch_gather3(dens_dev, P->sperm.p, P->ns, P->nsp, P->sx.p, P->sn.p, P->sa.p, P->sd.p, P->rec.p,
                          P->precision == HOR_MIXED ? P->rec32.p : nullptr, P->sub_box.p,
                          P->kind == HOR_SL_ON || P->kind == HOR_SL_OFF || P->kind == HOR_P_OFF, s));
@@ -360,9 +376,11 @@ extern "C" int hor_plan_create(hor_ctx* ctx, int kind, int method, int precision
   *plan = nullptr;
   const int flags = kind & 0xF0;
   kind &= 0x0F;
-  if (kind < HOR_SL_ON || kind > HOR_P_OFF) return set_error(HOR_E_ARG, "bad kind %d", kind);
+  if (kind < HOR_SL_ON || kind > HOR_SLDL_OFF) return set_error(HOR_E_ARG, "bad kind %d", kind);
   if (kind == HOR_P_OFF && (flags || method != HOR_DIRECT || precision != HOR_FP64))
     return set_error(HOR_E_ARG, "the pressure operator is a direct FP64 sum without kind flags");
+  if (kind == HOR_SLDL_OFF && (flags || method != HOR_DIRECT || precision != HOR_FP64))
+    return set_error(HOR_E_ARG, "the fused Stokeslet + stresslet operator is a direct FP64 sum without kind flags");
   if ((flags & ~(HOR_ALL_NEAR | HOR_ON_AT)) || ((flags & HOR_ALL_NEAR) && kind != HOR_SL_OFF && kind != HOR_DL_OFF) ||
       ((flags & HOR_ON_AT) && kind != HOR_DL_ON))
     return set_error(HOR_E_ARG, "bad kind flags 0x%x for kind %d", flags, kind);
@@ -375,7 +393,8 @@ extern "C" int hor_plan_create(hor_ctx* ctx, int kind, int method, int precision
   bool on = (kind == HOR_SL_ON || (kind == HOR_DL_ON && !(flags & HOR_ON_AT)));
   if (on && nt > ns) return set_error(HOR_E_ARG, "on-surface operator needs nt <= ns (target i is source i)");
   if (!on && (!tb || !tn0)) return set_error(HOR_E_ARG, "off-surface operator needs b and nrst normal");
-  if (kind == HOR_DL_OFF && !tx0) return set_error(HOR_E_ARG, "DL off-surface operator needs nrst point");
+  if ((kind == HOR_DL_OFF || kind == HOR_SLDL_OFF) && !tx0)
+    return set_error(HOR_E_ARG, "DL off-surface operator needs nrst point");
   if (method == HOR_TREE && !ctr) return set_error(HOR_E_ARG, "treecode needs the root-box centre");
   if (!all_finite(sx, 3 * (size_t)ns) || !all_finite(sn, 3 * (size_t)ns) || !all_finite(sa, ns) ||
       !all_finite(tx, 3 * (size_t)nt) || !all_finite(tb, tb ? nt : 0) ||
@@ -435,14 +454,15 @@ extern "C" int hor_plan_eval(hor_plan* P, const double* dens, const double* d0, 
   if (!P || !dens || !out) return set_error(HOR_E_ARG, "hor_plan_eval: null");
   bool on = (P->kind == HOR_SL_ON || P->kind == HOR_DL_ON) && !P->on_at;
   if (!on && !d0) return set_error(HOR_E_ARG, "off-surface operator needs d0 (nrst_f / nrst_g)");
-  if (!all_finite(dens, 3 * (size_t)P->ns) || (!on && !all_finite(d0, 3 * (size_t)P->nt)))
+  const size_t per = P->kind == HOR_SLDL_OFF ? 6 : 3;   // values per source / target
+  if (!all_finite(dens, per * (size_t)P->ns) || (!on && !all_finite(d0, per * (size_t)P->nt)))
     return set_error(HOR_E_NAN, "non-finite density");
   hipStream_t s = P->ctx->stream;
   HOR_HIP_TRY(hipSetDevice(P->ctx->device));
-  HOR_HIP_TRY(hipMemcpyAsync(P->in_dens.p, dens, 3 * sizeof(double) * P->ns, hipMemcpyHostToDevice, s));
-  if (!on) HOR_HIP_TRY(hipMemcpyAsync(P->in_d0.p, d0, 3 * sizeof(double) * P->nt, hipMemcpyHostToDevice, s));
+  HOR_HIP_TRY(hipMemcpyAsync(P->in_dens.p, dens, per * sizeof(double) * P->ns, hipMemcpyHostToDevice, s));
+  if (!on) HOR_HIP_TRY(hipMemcpyAsync(P->in_d0.p, d0, per * sizeof(double) * P->nt, hipMemcpyHostToDevice, s));
   HOR_TRY(hor_plan_eval_dev(P, P->in_dens.p, on ? nullptr : P->in_d0.p, P->out_buf.p));
-  const size_t nout = (P->kind == HOR_P_OFF ? 1 : 3) * (size_t)P->nt;
+  const size_t nout = (P->kind == HOR_P_OFF ? 1 : per) * (size_t)P->nt;
   HOR_HIP_TRY(hipMemcpyAsync(out, P->out_buf.p, sizeof(double) * nout, hipMemcpyDeviceToHost, s));
   HOR_HIP_TRY(hipStreamSynchronize(s));
   if (!all_finite(out, nout)) return set_error(HOR_E_NAN, "non-finite result");

@@ -14,7 +14,7 @@ _IP = C.POINTER(C.c_int)
 _DP = C.POINTER(C.c_double)
 
 HOR_OK = 0
-SL_ON, SL_OFF, DL_ON, DL_OFF, P_OFF = 0, 1, 2, 3, 4
+SL_ON, SL_OFF, DL_ON, DL_OFF, P_OFF, SLDL_OFF = 0, 1, 2, 3, 4, 5
 ALL_NEAR, ON_AT = 0x10, 0x20   # kind modifier bits (hor.h)
 DIRECT, TREE = 0, 1
 FP64, MIXED = 0, 1

@@ -14,9 +14,9 @@ import weakref
 import numpy as np
 
 from . import _lib
-from ._lib import DIRECT, DL_OFF, DL_ON, FP64, MIXED, P_OFF, SL_OFF, SL_ON, TREE, check, f64, ptr
+from ._lib import DIRECT, DL_OFF, DL_ON, FP64, MIXED, P_OFF, SL_OFF, SLDL_OFF, SL_ON, TREE, check, f64, ptr
 
-__all__ = ["Context", "Plan", "TreeParams", "SL_ON", "SL_OFF", "DL_ON", "DL_OFF", "P_OFF", "DIRECT", "TREE",
+__all__ = ["Context", "Plan", "TreeParams", "SL_ON", "SL_OFF", "DL_ON", "DL_OFF", "P_OFF", "SLDL_OFF", "DIRECT", "TREE",
            "FP64", "MIXED", "eval_Stokes_SL_offSurf", "eval_Stokes_SL_onSurf",
            "eval_Stokes_DL_offSurf", "eval_Stokes_DL_onSurf", "eval_Stokes_SL_onSurf_tree",
            "eval_Stokes_SL_offSurf_tree", "eval_Stokes_DL_onSurf_tree", "eval_Stokes_DL_offSurf_tree",
@@ -135,9 +135,12 @@ class Plan:
             pass
 
     def __call__(self, dens, d0=None):
-        dens = f64(dens, 3 * self.ns)
-        d0 = None if d0 is None else f64(d0, 3 * self.nt)
-        out = np.zeros((1 if self.kind == P_OFF else 3) * self.nt)
+        """SLDL_OFF: dens = [f | g] (6 ns), d0 = [f0 | g0] (6 nt); returns
+        [SL | DL] (6 nt)."""
+        per = 6 if self.kind == SLDL_OFF else 3
+        dens = f64(dens, per * self.ns)
+        d0 = None if d0 is None else f64(d0, per * self.nt)
+        out = np.zeros((1 if self.kind == P_OFF else per) * self.nt)
         check(_lib.lib().hor_plan_eval(self.h, ptr(dens), ptr(d0), ptr(out)), "hor_plan_eval")
         return out
 

@@ -44,9 +44,18 @@ enum {
   HOR_SL_OFF = 1,  /* eval_Stokes_SL_offSurf  (stokeslets.cpp:12-86)   */
   HOR_DL_ON = 2,   /* eval_Stokes_DL_onSurf   (stresslets.cpp:186-228) */
   HOR_DL_OFF = 3,  /* eval_Stokes_DL_offSurf  (stresslets.cpp:13-96)   */
-  HOR_P_OFF = 4    /* stokes_pressure_7ord (extend_to_grid/stokes.cpp:134-203):
+  HOR_P_OFF = 4,   /* stokes_pressure_7ord (extend_to_grid/stokes.cpp:134-203):
                       regularized pressure of the single layer with subtraction;
                       one scalar per target; direct method, FP64 only */
+  HOR_SLDL_OFF = 5 /* eval_Stokes_SL_offSurf and eval_Stokes_DL_offSurf at the
+                      same targets in one pass (the Stokeslet + stresslet
+                      velocity of a grid band): the pairs share the
+                      separation, 1/r and the erf/exp table; each kernel keeps
+                      its own expressions, so the two results are those of
+                      the SL_OFF and DL_OFF operators.  Direct method, FP64
+                      only; densities, nearest-point densities and results
+                      are the SL and DL arrays back to back (see
+                      hor_plan_eval) */
 };
 
 /* Modifier bits OR-ed into the kind of hor_plan_create (the reference's
@@ -106,6 +115,7 @@ typedef struct {
  *   tb: nt signed distances b (OFF only), tx0/tn0: 3*nt nearest surface point
  *       and its normal (OFF only; tx0 used by DL only).
  *   HOR_P_OFF takes the SL_OFF inputs; its result has one value per target.
+ *   HOR_SLDL_OFF takes the DL_OFF inputs.
  *   ctr: root-box centre for HOR_TREE (KITC.cpp:45-60), ignored otherwise.
  * ------------------------------------------------------------------------ */
 typedef struct hor_plan hor_plan;
@@ -121,12 +131,14 @@ int hor_plan_destroy(hor_plan* plan);
  *   dens: 3*ns source density (f for SL and P, g for DL), host.
  *   d0:   3*nt density at each target's nearest point (f0 = nrst_f for SL_OFF
  *         and P_OFF, g0 = nrst_g for DL_OFF), host; ignored for *_ON.
- *   out:  3*nt result (nt for P_OFF), host. */
+ *   out:  3*nt result (nt for P_OFF), host.
+ * HOR_SLDL_OFF: dens = [f | g] (6*ns), d0 = [f0 | g0] (6*nt),
+ *   out = [SL_OFF result | DL_OFF result] (6*nt). */
 int hor_plan_eval(hor_plan* plan, const double* dens, const double* d0, double* out);
 
 /* Device-resident variant: all three pointers are device pointers
  * (hor_dev_alloc); dens_dev 3*ns, d0_dev 3*nt or NULL, out_dev 3*nt (nt for
- * P_OFF). */
+ * P_OFF; 6*ns, 6*nt and 6*nt for SLDL_OFF). */
 int hor_plan_eval_dev(hor_plan* plan, const double* dens_dev, const double* d0_dev,
                       double* out_dev);
 
