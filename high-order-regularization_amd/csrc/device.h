@@ -158,6 +158,14 @@ struct __align__(16) SrcRecV {
   double f2, g0, g1, g2;
 };
 
+// FP32 copy of SrcRecV (mixed precision), position relative to the centre of
+// the source's 64-subtile box.
+struct __align__(16) SrcRecV32 {
+  float x, y, z, nx;
+  float ny, nz, f0, f1;
+  float f2, g0, g1, g2;
+};
+
 struct __align__(16) SrcRec32 {
   float x, y, z, a;
   float nx, ny, nz, pad0;

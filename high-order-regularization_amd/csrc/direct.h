@@ -22,6 +22,7 @@ struct DirectArgs {
   const double* sd;                // 3*nsp density (f or g)
   const struct SrcRec* rec;        // nsp source records (sorted), this evaluation's density
   const struct SrcRecV* recv;      // SLDL_OFF: nsp fused records
+  const struct SrcRecV32* recv32;  // SLDL_OFF mixed precision: FP32 copies (y - subtile centre)
   const struct SrcRec32* rec32;    // FP32 copies (mixed precision) or null; direct plans store
                                    // y - (centre of the source's 64-subtile box)
   const double* sub_box;           // 6 per 64-source subtile (lo xyz, hi xyz)
@@ -52,7 +53,8 @@ int launch_gather3(const double* dens, const int* perm, int n, int np, const dou
 int launch_sldl_finalize(const double* partial, int nchunk, int nt, int ntp, double scale_sl, double scale_dl,
                          const double* xi, const double* g0, const int* tperm, double* out, hipStream_t s);
 int launch_gather_sldl(const double* dens, const int* perm, int n, int np, const double* sx, const double* sn,
-                       const double* sa, struct SrcRecV* recv, hipStream_t s);
+                       const double* sa, struct SrcRecV* recv, struct SrcRecV32* recv32, const double* sub_box,
+                       hipStream_t s);
 int launch_target_sub(int mode, int nt, int ntp, const int* tperm, const int* tnear,
                       const double* d0, const double* n0_orig, const double* dens,
                       const double* sn_orig, double* out, hipStream_t s);

@@ -48,6 +48,7 @@ __global__ __launch_bounds__(kBlock) void k_direct(DirectArgs A) {
 
 // Stokeslet + stresslet at off-surface targets in one pass (HOR_SLDL_OFF):
 // the partial sums of both kernels, 6 per target per chunk.
+template <bool MIXED>
 __global__ __launch_bounds__(kBlock, 4) void k_direct_sldl(DirectArgs A) {
   const double* tab = stage_table<HOR_SL_OFF>(A);
   const int tid = threadIdx.x;
@@ -60,7 +61,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_direct_sldl(DirectArgs A) {
   double us[3] = {0.0, 0.0, 0.0}, ud[3] = {0.0, 0.0, 0.0};
   const int sub0 = blockIdx.y * A.subs_per_chunk;
   const int sub1 = min(A.nsub, sub0 + A.subs_per_chunk);
-  range_sum_sldl(A, sub0 * kWave, min(A.ns, sub1 * kWave), sub0, wb, all_far, t, tab, us, ud);
+  range_sum_sldl<MIXED>(A, sub0 * kWave, min(A.ns, sub1 * kWave), sub0, wb, all_far, t, tab, us, ud);
   double* P = A.partial + (size_t)blockIdx.y * 6 * A.ntp;
 #pragma unroll
   for (int k = 0; k < 3; k++) {
@@ -168,9 +169,11 @@ __global__ void k_gather3(const double* dens, const int* perm, int n, int np, co
 }
 
 // Fused records (HOR_SLDL_OFF), dens = [f | g]: recv[j] <- (y, a n, a f, g)
-// of sorted source j, the SL and DL records' values.
+// of sorted source j, the SL and DL records' values; with recv32 also their
+// FP32 copies, y relative to the centre of the source's 64-subtile box.
 __global__ void k_gather_sldl(const double* dens, const int* perm, int n, int np, const double* sx,
-                              const double* sn, const double* sa, SrcRecV* recv) {
+                              const double* sn, const double* sa, SrcRecV* recv, SrcRecV32* recv32,
+                              const double* sub_box) {
   int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
   int o = perm[j];
@@ -182,6 +185,17 @@ __global__ void k_gather_sldl(const double* dens, const int* perm, int n, int np
   r.f0 = dens[3 * o] * a; r.f1 = dens[3 * o + 1] * a; r.f2 = dens[3 * o + 2] * a;
   r.g0 = g[3 * o]; r.g1 = g[3 * o + 1]; r.g2 = g[3 * o + 2];
   recv[j] = r;
+  if (recv32) {
+    const double* bx = sub_box + 6 * (j / kWave);
+    SrcRecV32 q;
+    q.x = (float)(r.x - 0.5 * (bx[0] + bx[3]));
+    q.y = (float)(r.y - 0.5 * (bx[1] + bx[4]));
+    q.z = (float)(r.z - 0.5 * (bx[2] + bx[5]));
+    q.nx = (float)r.nx; q.ny = (float)r.ny; q.nz = (float)r.nz;
+    q.f0 = (float)r.f0; q.f1 = (float)r.f1; q.f2 = (float)r.f2;
+    q.g0 = (float)r.g0; q.g1 = (float)r.g1; q.g2 = (float)r.g2;
+    recv32[j] = q;
+  }
 }
 
 // Per-target subtraction data (sorted order):
@@ -249,8 +263,8 @@ __global__ void k_target_f0n0_all(int nt, int ntp, const int* tperm, const doubl
 int launch_direct(const DirectArgs& A, int kind, bool mixed, int nchunk, hipStream_t s) {
   dim3 grid(A.ntp / kBlock, nchunk), block(kBlock);
   if (kind == HOR_SLDL_OFF) {
-    if (mixed) return set_error(HOR_E_ARG, "the fused Stokeslet + stresslet sum is FP64 only");
-    hipLaunchKernelGGL(k_direct_sldl, grid, block, lds_bytes(HOR_SL_OFF), s, A);
+    if (mixed) hipLaunchKernelGGL(k_direct_sldl<true>, grid, block, lds_bytes(HOR_SL_OFF), s, A);
+    else hipLaunchKernelGGL(k_direct_sldl<false>, grid, block, lds_bytes(HOR_SL_OFF), s, A);
     HOR_HIP_TRY(hipGetLastError());
     return HOR_OK;
   }
@@ -297,9 +311,10 @@ int launch_sldl_finalize(const double* partial, int nchunk, int nt, int ntp, dou
 }
 
 int launch_gather_sldl(const double* dens, const int* perm, int n, int np, const double* sx, const double* sn,
-                       const double* sa, SrcRecV* recv, hipStream_t s) {
+                       const double* sa, SrcRecV* recv, SrcRecV32* recv32, const double* sub_box, hipStream_t s) {
   if (n == 0) return HOR_OK;
-  hipLaunchKernelGGL(k_gather_sldl, dim3((n + 255) / 256), dim3(256), 0, s, dens, perm, n, np, sx, sn, sa, recv);
+  hipLaunchKernelGGL(k_gather_sldl, dim3((n + 255) / 256), dim3(256), 0, s, dens, perm, n, np, sx, sn, sa, recv,
+                     recv32, sub_box);
   HOR_HIP_TRY(hipGetLastError());
   return HOR_OK;
 }

@@ -456,6 +456,82 @@ __device__ __forceinline__ void dl_off_reg32(const SrcRec32& s, float dx, float 
   u[2] += dz * Bd - v.n0z * Bn + xh2 * Bx;
 }
 
+// Stokeslet + stresslet (HOR_SLDL_OFF) in FP32: sl_plain32 / dl_plain32 and
+// sl_reg32 / dl_off_reg32 with the separation, 1/r, rho, exp(-rho^2), erf(rho)
+// and the s2 polynomial computed once.
+__device__ __forceinline__ void sldl_plain32(const SrcRecV32& s, float dx, float dy, float dz, const Tgt32& v,
+                                             float ws[3], float wd[3]) {
+  float r2 = dx * dx + dy * dy + dz * dz;
+  float ri = rsqrt32(r2);
+  float fx = s.f0 - v.sub * s.nx, fy = s.f1 - v.sub * s.ny, fz = s.f2 - v.sub * s.nz;
+  float tq = (fx * dx + fy * dy + fz * dz) * (ri * ri);
+  ws[0] += (fx + dx * tq) * ri;
+  ws[1] += (fy + dy * tq) * ri;
+  ws[2] += (fz + dz * tq) * ri;
+  float ri2 = ri * ri;
+  float gdx = (s.g0 - v.g0x) * dx + (s.g1 - v.g0y) * dy + (s.g2 - v.g0z) * dz;
+  float ndx = s.nx * dx + s.ny * dy + s.nz * dz;
+  float k = gdx * ndx * (ri2 * ri2 * ri);
+  wd[0] += dx * k;
+  wd[1] += dy * k;
+  wd[2] += dz * k;
+}
+
+__device__ __forceinline__ void sldl_reg32(const SrcRecV32& s, float dx, float dy, float dz, const Tgt32& v,
+                                           bool nearp, float inv_del, float ws[3], float wd[3]) {
+  const float r2 = dx * dx + dy * dy + dz * dz;
+  const float ri = rsqrt32(r2);
+  const float rd = (r2 * ri) * inv_del;
+  const float ri2 = ri * ri;
+  const float ri3 = ri * ri * ri;
+  const float fx = s.f0 - v.sub * s.nx, fy = s.f1 - v.sub * s.ny, fz = s.f2 - v.sub * s.nz;
+  const float fdx = fx * dx + fy * dy + fz * dz;
+  const float gx = s.g0 - v.g0x, gy = s.g1 - v.g0y, gz = s.g2 - v.g0z;
+  if (!nearp || rd > 8.0f) {
+    const float HS = ri3 * fdx;
+    ws[0] += fx * ri + dx * HS;
+    ws[1] += fy * ri + dy * HS;
+    ws[2] += fz * ri + dz * HS;
+    const float gdx = gx * dx + gy * dy + gz * dz;
+    const float ndx = s.nx * dx + s.ny * dy + s.nz * dz;
+    const float k = gdx * ndx * (ri2 * ri2 * ri);
+    wd[0] += dx * k;
+    wd[1] += dy * k;
+    wd[2] += dz * k;
+    return;
+  }
+  const float rd2 = rd * rd;
+  const float E = expneg32(rd2);
+  const float ef = erf_tail32(rd, E);
+  const float rex = rd * E;
+  const float p1 = v.q[0] + rd2 * (v.q[1] + rd2 * v.q[2]);
+  const float p2 = v.q[3] + rd2 * (v.q[4] + rd2 * (v.q[5] + rd2 * v.q[6]));
+  const float p3 = v.q[7] + rd2 * (v.q[8] + rd2 * (v.q[9] + rd2 * (v.q[10] + rd2 * v.q[11])));
+  const float s1 = (ef + p1 * rex) * ri;
+  const float s2 = (ef + p2 * rex) * ri3;
+  const float HS = s2 * fdx;
+  ws[0] += fx * s1 + dx * HS;
+  ws[1] += fy * s1 + dy * HS;
+  ws[2] += fz * s1 + dz * HS;
+  const float H2 = (ef + p2 * rex) * (ri2 * ri);
+  const float H3 = (ef + p3 * rex) * ((ri2 * ri) * ri2);
+  const float xh0 = v.ex - dx, xh1 = v.ey - dy, xh2 = v.ez - dz;
+  const float sig2 = (xh0 * xh0 + xh1 * xh1 + xh2 * xh2) - 2.0f * v.b * (xh0 * v.n0x + xh1 * v.n0y + xh2 * v.n0z);
+  const float g_n0 = gx * v.n0x + gy * v.n0y + gz * v.n0z;
+  const float g_xh = gx * xh0 + gy * xh1 + gz * xh2;
+  const float n0_n = v.n0x * s.nx + v.n0y * s.ny + v.n0z * s.nz;
+  const float xh_n = xh0 * s.nx + xh1 * s.ny + xh2 * s.nz;
+  const float m1a = g_n0 * n0_n;
+  const float m1b = g_xh * n0_n + g_n0 * xh_n;
+  const float m2a = g_xh * xh_n;
+  const float m2b = v.b * m1b;
+  const float A1 = H2 - sig2 * H3;
+  const float Bd = m1a * A1 + m2a * H3, Bn = m1b * A1, Bx = m2b * H3;
+  wd[0] += dx * Bd - v.n0x * Bn + xh0 * Bx;
+  wd[1] += dy * Bd - v.n0y * Bn + xh1 * Bx;
+  wd[2] += dz * Bd - v.n0z * Bn + xh2 * Bx;
+}
+
 __device__ __forceinline__ void dl_on_reg32(const SrcRec32& s, float dx, float dy, float dz, const Tgt32& v,
                                             float inv_del, float u[3]) {
   const float r2 = dx * dx + dy * dy + dz * dz;
@@ -650,9 +726,13 @@ __device__ __forceinline__ void range_sum(const DirectArgs& A, int j0, int j1, i
   }
 }
 
-// Stokeslet + stresslet over the sorted sources [j0, j1) (HOR_SLDL_OFF, FP64):
-// the subtile test, the per-source vote and the pair functions of the
-// separate SL_OFF and DL_OFF sums, in the same order.
+// Stokeslet + stresslet over the sorted sources [j0, j1) (HOR_SLDL_OFF): the
+// subtile test, the per-source vote and the pair functions of the separate
+// SL_OFF and DL_OFF sums (range_sum), in the same order.  MIXED: far subtiles
+// in FP32 on subtile-centred coordinates; in a mixed subtile FP64 if some lane
+// is within 4 delta, FP32 regularized if within 8 delta, FP64 plain otherwise;
+// FP32 terms summed per subtile in FP32 and the subtiles in FP64.
+template <bool MIXED>
 __device__ __forceinline__ void range_sum_sldl(const DirectArgs& A, int j0, int j1, int sub0, const double* wb,
                                                bool all_far, const Tgt& t, const double* __restrict__ tab,
                                                double us[3], double ud[3]) {
@@ -662,20 +742,51 @@ __device__ __forceinline__ void range_sum_sldl(const DirectArgs& A, int j0, int 
   const double self1 = 32.0 / 5.0 / rp / A.DEL;
   const double self2 = 256.0 / 15.0 / rp / (A.DEL * A.DEL * A.DEL);
   const SrcRecV* __restrict__ R = A.recv;
+  const SrcRecV32* __restrict__ R32 = A.recv32;
+  Tgt32 v;
+  if (MIXED) make_tgt32(t, v);
+  const float inv_del32 = (float)inv_del;
   for (int base = j0, sb = sub0; base < j1; base += kWave, sb++) {
     const int end = min(j1, base + kWave);
     const double* box = A.sub_box + 6 * sb;
     if (all_far || boxes_far(wb, box, A.far_d2)) {
+      if (MIXED) {
+        const float cx = (float)(t.px - 0.5 * (box[0] + box[3]));
+        const float cy = (float)(t.py - 0.5 * (box[1] + box[4]));
+        const float cz = (float)(t.pz - 0.5 * (box[2] + box[5]));
+        float ws[3] = {0.0f, 0.0f, 0.0f}, wd[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll 2
-      for (int j = base; j < end; j++) sldl_plain(R[j], t, us, ud);
+        for (int j = base; j < end; j++) {
+          const SrcRecV32& s = R32[j];
+          sldl_plain32(s, cx - s.x, cy - s.y, cz - s.z, v, ws, wd);
+        }
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+          us[k] += (double)ws[k];
+          ud[k] += (double)wd[k];
+        }
+      } else {
+#pragma unroll 2
+        for (int j = base; j < end; j++) sldl_plain(R[j], t, us, ud);
+      }
     } else {
+      float ws[3] = {0.0f, 0.0f, 0.0f}, wd[3] = {0.0f, 0.0f, 0.0f};
       for (int j = base; j < end; j++) {
         const SrcRecV s = R[j];
         const double dx = t.px - s.x, dy = t.py - s.y, dz = t.pz - s.z;
         const double r2 = dx * dx + dy * dy + dz * dz;
         const bool need = t.nearp && !(r2 > A.near_r2);
-        if (__any(need ? 1 : 0)) sldl_full(s, t, inv_del, inv_del2, self1, self2, tab, us, ud);
+        if (MIXED && !__any((t.nearp && !(r2 > A.near64_r2)) ? 1 : 0) && __any(need ? 1 : 0))
+          sldl_reg32(R32[j], (float)dx, (float)dy, (float)dz, v, t.nearp, inv_del32, ws, wd);
+        else if (__any(need ? 1 : 0)) sldl_full(s, t, inv_del, inv_del2, self1, self2, tab, us, ud);
         else sldl_plain(s, t, us, ud);
+      }
+      if (MIXED) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+          us[k] += (double)ws[k];
+          ud[k] += (double)wd[k];
+        }
       }
     }
   }

@@ -260,9 +260,13 @@ int setup_sources(hor_plan* P, const std::vector<int>& perm, const double* sx, c
   HOR_TRY(P->sa.upload(A, s));
   HOR_TRY(P->sub_box.upload(SB, s));
   HOR_TRY(P->sd.alloc(3 * (size_t)nsp));
-  if (P->kind == HOR_SLDL_OFF) HOR_TRY(P->recv.alloc(nsp));
-  else HOR_TRY(P->rec.alloc(nsp));
-  if (P->precision == HOR_MIXED) HOR_TRY(P->rec32.alloc(nsp));
+  if (P->kind == HOR_SLDL_OFF) {
+    HOR_TRY(P->recv.alloc(nsp));
+    if (P->precision == HOR_MIXED) HOR_TRY(P->recv32.alloc(nsp));
+  } else {
+    HOR_TRY(P->rec.alloc(nsp));
+    if (P->precision == HOR_MIXED) HOR_TRY(P->rec32.alloc(nsp));
+  }
   HOR_TRY(P->sn_orig.upload(sn, 3 * (size_t)ns, s));
   HOR_TRY(P->in_dens.alloc((P->kind == HOR_SLDL_OFF ? 6 : 3) * (size_t)ns));
   return HOR_OK;
@@ -281,6 +285,7 @@ DirectArgs direct_args(hor_plan* P) {
   A.near64_r2 = 16.0 * P->DEL * P->DEL;
   A.sx = P->sx.p; A.sn = P->sn.p; A.sa = P->sa.p; A.sd = P->sd.p; A.sub_box = P->sub_box.p;
   A.rec = P->rec.p; A.recv = P->recv.p; A.rec32 = P->precision == HOR_MIXED ? P->rec32.p : nullptr;
+  A.recv32 = P->precision == HOR_MIXED ? P->recv32.p : nullptr;
   A.tx = P->tx.p; A.tcoef = P->tcoef.p; A.tsub = P->tsub.p;
   A.tx0 = P->tx0.p; A.tn0 = P->tn0.p; A.tb = P->tb.p;
   A.tnear = P->tnear.p; A.wave_box = P->wave_box.p; A.partial = P->partial.p;
@@ -332,11 +337,13 @@ static int plan_eval_direct(hor_plan* P, const double* dens_dev, const double* d
   if (P->kind == HOR_SLDL_OFF) {
     if (base || coef != 1.0 || accumulate)
       return set_error(HOR_E_ARG, "the fused Stokeslet + stresslet sum writes its result only");
-    HOR_TRY(launch_gather_sldl(dens_dev, P->sperm.p, P->ns, P->nsp, P->sx.p, P->sn.p, P->sa.p, P->recv.p, s));
+    const bool mixed = P->precision == HOR_MIXED;
+    HOR_TRY(launch_gather_sldl(dens_dev, P->sperm.p, P->ns, P->nsp, P->sx.p, P->sn.p, P->sa.p, P->recv.p,
+                               mixed ? P->recv32.p : nullptr, P->sub_box.p, s));
     HOR_TRY(prepare_targets(P, dens_dev, d0_dev));
     DirectArgs A = direct_args(P);
     HOR_HIP_TRY(hipEventRecord(c->ev[2], s));
-    HOR_TRY(launch_direct(A, P->kind, false, P->nchunk, s));
+    HOR_TRY(launch_direct(A, P->kind, mixed, P->nchunk, s));
     HOR_HIP_TRY(hipEventRecord(c->ev[3], s));
     return launch_sldl_finalize(P->partial.p, P->nchunk, P->nt, P->ntp, op_scale(HOR_SL_OFF), op_scale(HOR_DL_OFF),
                                 P->xi.p, P->tsub.p + P->ntp, P->tperm.p, out_dev, s);
@@ -379,8 +386,8 @@ extern "C" int hor_plan_create(hor_ctx* ctx, int kind, int method, int precision
   if (kind < HOR_SL_ON || kind > HOR_SLDL_OFF) return set_error(HOR_E_ARG, "bad kind %d", kind);
   if (kind == HOR_P_OFF && (flags || method != HOR_DIRECT || precision != HOR_FP64))
     return set_error(HOR_E_ARG, "the pressure operator is a direct FP64 sum without kind flags");
-  if (kind == HOR_SLDL_OFF && (flags || method != HOR_DIRECT || precision != HOR_FP64))
-    return set_error(HOR_E_ARG, "the fused Stokeslet + stresslet operator is a direct FP64 sum without kind flags");
+  if (kind == HOR_SLDL_OFF && (flags || method != HOR_DIRECT))
+    return set_error(HOR_E_ARG, "the fused Stokeslet + stresslet operator is a direct sum without kind flags");
   if ((flags & ~(HOR_ALL_NEAR | HOR_ON_AT)) || ((flags & HOR_ALL_NEAR) && kind != HOR_SL_OFF && kind != HOR_DL_OFF) ||
       ((flags & HOR_ON_AT) && kind != HOR_DL_ON))
     return set_error(HOR_E_ARG, "bad kind flags 0x%x for kind %d", flags, kind);

@@ -53,7 +53,7 @@ enum {
                       separation, 1/r and the erf/exp table; each kernel keeps
                       its own expressions, so the two results are those of
                       the SL_OFF and DL_OFF operators.  Direct method, FP64
-                      only; densities, nearest-point densities and results
+                      or mixed; densities, nearest-point densities and results
                       are the SL and DL arrays back to back (see
                       hor_plan_eval) */
 };

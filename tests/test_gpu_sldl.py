@@ -9,7 +9,9 @@ same off-surface targets in one pass.
   the shared separation and 1/r are computed once);
 * targets on both sides of the surface and beyond 8 delta (plain class), a
   target count that is not a multiple of the 256-target block;
-* the operator is direct FP64 only.
+* mixed precision: within 1e-6 of FP64 and 1e-7 (FP32 rounding) of the
+  separate mixed operators;
+* the operator is a direct sum only.
 """
 import numpy as np
 import pytest
@@ -62,9 +64,26 @@ def test_fused_far_targets_and_ragged_count(ctx, oracle):
     _check(fdl, dl, 1e-15)
 
 
-def test_fused_is_direct_fp64_only(ctx):
+def test_fused_mixed_precision(ctx):
+    """Mixed precision (FP32 far pairs and 4-8 delta regularized pairs, FP64
+    accumulation): within 1e-6 of FP64, like the separate mixed operators,
+    and within FP32 rounding (1e-7) of them: the same FP32 expressions, which
+    the compiler may contract differently."""
+    body, s, t, DEL = ellipsoid_case(h=1 / 32, max_band=6001)
+    n = t["n"]
+    fg, fg0 = np.concatenate([s["f"], s["g"]]), np.concatenate([t["f0"], t["g0"]])
+    out64 = hor.Plan(ctx, hor.SLDL_OFF, s, t, DEL)(fg, fg0)
+    out32 = hor.Plan(ctx, hor.SLDL_OFF, s, t, DEL, precision=hor.MIXED)(fg, fg0)
+    for half, kind, dens, d0 in ((slice(0, 3 * n), hor.SL_OFF, s["f"], t["f0"]),
+                                 (slice(3 * n, 6 * n), hor.DL_OFF, s["g"], t["g0"])):
+        sep32 = hor.Plan(ctx, kind, s, t, DEL, precision=hor.MIXED)(dens, d0)
+        print("fused mixed", kind, "vs fp64", rel_l2(out32[half], out64[half]), "vs separate mixed",
+              rel_l2(out32[half], sep32), flush=True)
+        assert rel_l2(out32[half], out64[half]) <= 1e-6
+        assert rel_l2(out32[half], sep32) <= 1e-7
+
+
+def test_fused_is_direct_only(ctx):
     body, s, t, DEL = ellipsoid_case(h=1 / 8, max_band=200)
-    for method, prec in ((hor.TREE, hor.FP64), (hor.DIRECT, hor.MIXED)):
-        with pytest.raises(hor.HorError):
-            hor.Plan(ctx, hor.SLDL_OFF, s, t, DEL, method=method, precision=prec, ctr=body.ctr,
-                     tree=hor.TreeParams(4, 100, 0.6))
+    with pytest.raises(hor.HorError):
+        hor.Plan(ctx, hor.SLDL_OFF, s, t, DEL, method=hor.TREE, ctr=body.ctr, tree=hor.TreeParams(4, 100, 0.6))
