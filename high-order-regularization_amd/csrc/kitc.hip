@@ -108,61 +108,80 @@ __global__ void k_moment_rhs(int ns, int nsp, int dl, const double* sn, const do
 typedef double f64x4_t __attribute__((ext_vector_type(4)));
 
 // Proxy weights (Panel_Moments_*, KITC.cpp:351-668) as a GEMM on the FP64
-// matrix cores: W[kk][c] = sum_s B[kk][s] Q[s][c] with B[kk][s] =
-// a1_i a2_j a3_k D (kk = (i, j, k)) formed in registers and Q = A q.  A work
-// item is a far node's member chunk; each wave owns 16 proxies and runs
-// through the chunk four members per v_mfma_f64_16x16x4f64 (lane l holds
-// B[l%16][l/16] and Q[l/16][l%16]; accumulator r of lane l is
-// W[l/16 + 4 r][l%16]), alternating two accumulators so that consecutive
-// products do not wait on each other.
+// matrix cores.  With kk = (i, j, k) the weights of component c factor as
+//   W[(j,k)][(i,c)] = sum_s A[(j,k)][s] B[s][(i,c)],
+//   A[(j,k)][s] = a2_j a3_k,   B[s][(i,c)] = a1_i D q_c   (q = A q of member s),
+// a GEMM of np1^2 rows and np1*ncomp columns, every column a weight.  A work
+// item is a far node's member chunk; each wave owns 16 rows and all column
+// tiles (at most kMomTiles) and runs through the chunk four members per
+// v_mfma_f64_16x16x4f64: lane l holds A[l%16][l/16] (one value per step,
+// shared by the tiles) and B[l/16][16 t + l%16] for tile t; accumulator r of
+// tile t in lane l is W[l/16 + 4 r][16 t + l%16].  The tiles' products are
+// independent, so consecutive MFMAs do not wait on each other.
+constexpr int kMomTiles = 8;   // np1*ncomp <= 13*9 = 117 columns (P <= 12)
+
+// One step of four members for the NT column tiles (IN: some members of the
+// step are past the chunk and contribute zero).
+template <int NT, bool IN>
+__device__ __forceinline__ void moment_step(const double* fm, const double* qm, int aj, int ak, int dj,
+                                            const int* bi, const int* bc, bool in, f64x4_t* acc) {
+  const double a = (!IN || in) ? fm[aj] * fm[ak] : 0.0;
+  const double d = (!IN || in) ? fm[dj] : 0.0;
+#pragma unroll
+  for (int t = 0; t < NT; t++) {
+    const double b = (!IN || in) ? fm[bi[t]] * d * qm[bc[t]] : 0.0;
+    acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[t], 0, 0, 0);
+  }
+}
+
+template <int NT>
 __global__ __launch_bounds__(kBlock) void k_moments(MomArgs A) {
   const int item = blockIdx.x;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int kk0 = (blockIdx.y * 4 + wave) * 16;
-  if (kk0 >= A.pflat) return;  // wave-uniform
+  const int np1 = A.np1, ncomp = A.ncomp, nrow = np1 * np1, ncol = np1 * ncomp;
+  const int row0 = (blockIdx.y * 4 + wave) * 16;
+  if (row0 >= nrow) return;  // wave-uniform
   const int f = A.item_far[item];
   const int node = A.far_node[f];
   const int lo = A.node_lo[node];
   const int c0 = A.item_m0[item], c1 = A.item_m1[item];  // member range within the node
-  const int np1 = A.np1, stride = 3 * np1 + 1;
-  // rows past pflat compute a clamped proxy's weights and are not stored
-  const int row = min(kk0 + (lane & 15), A.pflat - 1);
-  const int i = row / (np1 * np1), j = (row / np1) % np1, k = row % np1;
-  const int ms = lane >> 4, col = lane & 15;
-  const double* fm = A.fac + A.fac_off[f] + (size_t)(c0 + ms) * stride;
-  const double* qm = A.mq + (size_t)(lo + c0 + ms) * 16 + col;
-  const size_t fstep = (size_t)4 * stride;
-  f64x4_t acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
-  const int n = c1 - c0;
-  const int nmain = n & ~3;  // members in whole groups of four: no bounds tests
-  int m0 = 0;
-  for (; m0 + 8 <= nmain; m0 += 8) {
-    const double* g = fm + fstep;
-    const double b0 = fm[i] * fm[np1 + j] * fm[2 * np1 + k] * fm[3 * np1];
-    const double b1 = g[i] * g[np1 + j] * g[2 * np1 + k] * g[3 * np1];
-    const double q0 = qm[0], q1 = qm[64];
-    acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, q0, acc0, 0, 0, 0);
-    acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, q1, acc1, 0, 0, 0);
-    fm += 2 * fstep;
-    qm += 128;
+  const int stride = 3 * np1 + 1;
+  const int ms = lane >> 4, l16 = lane & 15;
+  // the lane's A row (j, k) and, per tile, its B column (i, c); rows and
+  // columns past the ends use the last one and are not stored
+  const int ar = min(row0 + l16, nrow - 1);
+  const int aj = np1 + ar / np1, ak = 2 * np1 + ar % np1, dj = 3 * np1;
+  int bi[NT], bc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; t++) {
+    const int col = min(16 * t + l16, ncol - 1);
+    bi[t] = col / ncomp;
+    bc[t] = col - bi[t] * ncomp;
   }
-  if (m0 < nmain) {
-    const double b0 = fm[i] * fm[np1 + j] * fm[2 * np1 + k] * fm[3 * np1];
-    acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, qm[0], acc0, 0, 0, 0);
-    fm += fstep;
+  f64x4_t acc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; t++) acc[t] = f64x4_t{0.0, 0.0, 0.0, 0.0};
+  const double* fm = A.fac + A.fac_off[f] + (size_t)(c0 + ms) * stride;
+  const double* qm = A.mq + (size_t)(lo + c0 + ms) * 16;
+  const int n = c1 - c0;
+  const int nmain = n & ~3;  // whole steps of four members: no bounds tests
+  for (int m0 = 0; m0 < nmain; m0 += 4) {
+    moment_step<NT, false>(fm, qm, aj, ak, dj, bi, bc, true, acc);
+    fm += (size_t)4 * stride;
     qm += 64;
   }
-  if (nmain < n) {  // the last 1-3 members; the other lanes contribute zero
-    const bool in = nmain + ms < n;
-    const double bv = in ? fm[i] * fm[np1 + j] * fm[2 * np1 + k] * fm[3 * np1] : 0.0;
-    const double qv = in ? *qm : 0.0;
-    acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(bv, qv, acc1, 0, 0, 0);
-  }
-  double* P = A.part + (size_t)item * A.pflat * A.ncomp;
+  if (nmain < n) moment_step<NT, true>(fm, qm, aj, ak, dj, bi, bc, nmain + ms < n, acc);
+  double* P = A.part + (size_t)item * A.pflat * ncomp;
 #pragma unroll
-  for (int r = 0; r < 4; r++) {
-    const int kk = kk0 + (lane >> 4) + 4 * r;
-    if (kk < A.pflat && col < A.ncomp) P[(size_t)kk * A.ncomp + col] = acc0[r] + acc1[r];
+  for (int t = 0; t < NT; t++) {
+    const int col = 16 * t + l16;
+    if (col >= ncol) continue;
+    const int i = col / ncomp, c = col - i * ncomp;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int row = row0 + ms + 4 * r;   // j np1 + k
+      if (row < nrow) P[((size_t)i * nrow + row) * ncomp + c] = acc[t][r];
+    }
   }
 }
 
@@ -696,8 +715,18 @@ int tree_plan_eval(hor_plan* P, const double* dens_dev, const double* d0_dev, do
     M.item_far = TP.mom_item_far.p; M.item_m0 = TP.mom_item_m0.p; M.item_m1 = TP.mom_item_m1.p;
     M.far_node = TP.far_node.p; M.node_lo = TP.node_lo.p; M.fac_off = TP.fac_off.p;
     M.fac = TP.fac.p; M.mq = TP.mq.p; M.part = TP.mom_part.p;
-    const int tiles = (TP.pflat + 15) / 16;
-    hipLaunchKernelGGL(k_moments, dim3(TP.n_mom_items, (tiles + 3) / 4), dim3(kBlock), 0, s, M);
+    const int rows = TP.np1 * TP.np1;   // 16 rows per wave, 4 waves per workgroup
+    const dim3 mg(TP.n_mom_items, (rows + 63) / 64);
+    switch ((TP.np1 * TP.ncomp + 15) / 16) {   // column tiles, 1..kMomTiles for P <= 12
+      case 1: hipLaunchKernelGGL(k_moments<1>, mg, dim3(kBlock), 0, s, M); break;
+      case 2: hipLaunchKernelGGL(k_moments<2>, mg, dim3(kBlock), 0, s, M); break;
+      case 3: hipLaunchKernelGGL(k_moments<3>, mg, dim3(kBlock), 0, s, M); break;
+      case 4: hipLaunchKernelGGL(k_moments<4>, mg, dim3(kBlock), 0, s, M); break;
+      case 5: hipLaunchKernelGGL(k_moments<5>, mg, dim3(kBlock), 0, s, M); break;
+      case 6: hipLaunchKernelGGL(k_moments<6>, mg, dim3(kBlock), 0, s, M); break;
+      case 7: hipLaunchKernelGGL(k_moments<7>, mg, dim3(kBlock), 0, s, M); break;
+      default: hipLaunchKernelGGL(k_moments<kMomTiles>, mg, dim3(kBlock), 0, s, M); break;
+    }
     hipLaunchKernelGGL(k_moments_reduce, dim3(TP.nfar_nodes), dim3(kBlock), 0, s, TP.ncomp, TP.pflat,
                        TP.mom_node_first.p, TP.mom_node_nitems.p, TP.mom_part.p, TP.mom.p,
                        mixed ? TP.mom32.p : nullptr, dl ? kNc32Dl : kNc32Sl);
