@@ -47,9 +47,11 @@ int launch_finalize(const double* partial, int nchunk, int nt, int ntp, double s
 int launch_pressure_finalize(const double* partial, int nchunk, int nt, int ntp, double scale, const double* xi,
                              const double* f0n0, const int* tperm, double* out, const double* base, double coef,
                              int accumulate, hipStream_t s);
+// rec32 positions are relative to the centre of box sub_of[j] of sub_box
+// (sub_of NULL: box j/64), or absolute when sub_box is NULL.
 int launch_gather3(const double* dens, const int* perm, int n, int np, const double* sx, const double* sn,
                    const double* sa, double* sd, struct SrcRec* rec, struct SrcRec32* rec32,
-                   const double* sub_box, int scale_density, hipStream_t s);
+                   const double* sub_box, int scale_density, hipStream_t s, const int* sub_of = nullptr);
 int launch_sldl_finalize(const double* partial, int nchunk, int nt, int ntp, double scale_sl, double scale_dl,
                          const double* xi, const double* g0, const int* tperm, double* out, hipStream_t s);
 int launch_gather_sldl(const double* dens, const int* perm, int n, int np, const double* sx, const double* sn,

@@ -135,7 +135,7 @@ __global__ void k_direct_finalize(const double* partial, int nchunk, int nt, int
 // treecode moments: rec[j] <- geometry[j], dens[3*perm[j] + k].
 __global__ void k_gather3(const double* dens, const int* perm, int n, int np, const double* sx, const double* sn,
                           const double* sa, double* sd, SrcRec* rec, SrcRec32* rec32, const double* sub_box,
-                          int scale_density) {
+                          int scale_density, const int* sub_of) {
   int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
   int o = perm[j];
@@ -153,11 +153,12 @@ __global__ void k_gather3(const double* dens, const int* perm, int n, int np, co
   }
   rec[j] = r;
   if (rec32) {
-    // positions relative to the centre of the source's 64-subtile box (the
-    // direct sums' FP32 plain loop), absolute otherwise
+    // positions relative to the centre of the source's subtile box (the FP32
+    // plain loops: the direct sums' 64-source subtiles, the treecode's leaf
+    // subtiles through sub_of), absolute otherwise
     double c[3] = {0.0, 0.0, 0.0};
     if (sub_box) {
-      const double* b = sub_box + 6 * (j / kWave);
+      const double* b = sub_box + 6 * (sub_of ? sub_of[j] : j / kWave);
       for (int k = 0; k < 3; k++) c[k] = 0.5 * (b[k] + b[3 + k]);
     }
     SrcRec32 q;
@@ -331,10 +332,10 @@ int launch_pressure_finalize(const double* partial, int nchunk, int nt, int ntp,
 
 int launch_gather3(const double* dens, const int* perm, int n, int np, const double* sx, const double* sn,
                    const double* sa, double* sd, SrcRec* rec, SrcRec32* rec32, const double* sub_box,
-                   int scale_density, hipStream_t s) {
+                   int scale_density, hipStream_t s, const int* sub_of) {
   if (n == 0) return HOR_OK;
   hipLaunchKernelGGL(k_gather3, dim3((n + 255) / 256), dim3(256), 0, s, dens, perm, n, np, sx, sn, sa, sd, rec, rec32,
-                     sub_box, scale_density);
+                     sub_box, scale_density, sub_of);
   HOR_HIP_TRY(hipGetLastError());
   return HOR_OK;
 }

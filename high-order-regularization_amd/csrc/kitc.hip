@@ -36,6 +36,7 @@ struct TreePlan {
   int nslot = 0;
   // node data on the device
   DBuf<int> node_lo, node_hi, leaf_sub_off, far_idx;
+  DBuf<int> slot_sub;        // per tree slot: the leaf subtile holding it (FP32 record centre)
   DBuf<double> cheb, leaf_box;
   // near work items
   int n_near_items = 0;
@@ -244,7 +245,7 @@ __global__ __launch_bounds__(kBlock) void k_leaf(LeafArgs A) {
   const bool all_far = off_kind && !__any(t.nearp ? 1 : 0);
   double u[3] = {0.0, 0.0, 0.0};
   const int lo = A.node_lo[node], hi = A.node_hi[node];
-  range_sum<KIND, MIXED, false>(A.D, lo, hi + 1, A.leaf_sub_off[node], wb, all_far, t, tab, u);
+  range_sum<KIND, MIXED, MIXED>(A.D, lo, hi + 1, A.leaf_sub_off[node], wb, all_far, t, tab, u);
   if (act) {
     int s = A.slot[li];
     A.slots[s] = u[0];
@@ -614,6 +615,14 @@ int tree_plan_create(hor_plan* P, const double* sx, const double* sn, const doub
   HOR_TRY(TP.node_hi.upload(hi, s));
   HOR_TRY(TP.leaf_sub_off.upload(lso, s));
   HOR_TRY(TP.leaf_box.upload(lbox, s));
+  {
+    // each slot's leaf subtile: nodes are numbered parents first, so the last
+    // node holding a slot is its leaf
+    std::vector<int> ssub(P->nsp, 0);
+    for (int q = 0; q < nn; q++)
+      for (int j = lo[q]; j <= hi[q]; j++) ssub[j] = lso[q] + (j - lo[q]) / kWave;
+    HOR_TRY(TP.slot_sub.upload(ssub, s));
+  }
   HOR_TRY(TP.cheb.upload(T.cheb, s));
   {
     // node centres (midpoints of the Chebyshev-Lobatto intervals) and the
@@ -695,8 +704,8 @@ int tree_plan_eval(hor_plan* P, const double* dens_dev, const double* d0_dev, do
   const bool dl = (P->kind == HOR_DL_ON || P->kind == HOR_DL_OFF);
   // density in tree order; per-target subtraction data (leaf and proxy variants)
   HOR_TRY(launch_gather3(dens_dev, P->sperm.p, P->ns, P->nsp, P->sx.p, P->sn.p, P->sa.p, P->sd.p, P->rec.p,
-                         mixed ? P->rec32.p : nullptr, nullptr,
-                         P->kind == HOR_SL_ON || P->kind == HOR_SL_OFF, s));
+                         mixed ? P->rec32.p : nullptr, TP.leaf_box.p,
+                         P->kind == HOR_SL_ON || P->kind == HOR_SL_OFF, s, TP.slot_sub.p));
   HOR_TRY(prepare_targets(P, dens_dev, d0_dev));
   if (P->kind == HOR_SL_OFF)
     HOR_TRY(launch_target_f0n0_all(P->nt, P->ntp, P->tperm.p, d0_dev, P->n0_orig.p, P->tsub2.p, s));
