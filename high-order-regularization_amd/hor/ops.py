@@ -172,6 +172,15 @@ def eval_Stokes_DL_offSurf(ctx, h, DEL, surf, targets):
     return Plan(ctx, DL_OFF, surf, targets, DEL)(surf["g"], targets["g0"])
 
 
+def eval_Stokes_SL_DL_offSurf(ctx, h, DEL, surf, targets, precision=FP64):
+    """eval_Stokes_SL_offSurf and eval_Stokes_DL_offSurf at the same targets in
+    one pass (HOR_SLDL_OFF); returns (SL, DL), each 3*nt."""
+    n = targets["n"]
+    out = Plan(ctx, SLDL_OFF, surf, targets, DEL, precision=precision)(
+        np.concatenate([surf["f"], surf["g"]]), np.concatenate([targets["f0"], targets["g0"]]))
+    return out[:3 * n], out[3 * n:]
+
+
 def eval_Stokes_DL_onSurf(ctx, DEL, surf, targets=None):
     """stresslets.cpp:186-228."""
     return Plan(ctx, DL_ON, surf, targets, DEL)(surf["g"])

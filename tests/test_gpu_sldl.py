@@ -31,13 +31,10 @@ def _check(a, b, tol=TOL):
 
 
 def _fused_and_separate(ctx, s, t, DEL):
-    n = t["n"]
-    P = hor.Plan(ctx, hor.SLDL_OFF, s, t, DEL)
-    out = P(np.concatenate([s["f"], s["g"]]), np.concatenate([t["f0"], t["g0"]]))
-    P.close()
-    sl = hor.Plan(ctx, hor.SL_OFF, s, t, DEL)(s["f"], t["f0"])
-    dl = hor.Plan(ctx, hor.DL_OFF, s, t, DEL)(s["g"], t["g0"])
-    return out[:3 * n], out[3 * n:], sl, dl
+    fsl, fdl = hor.ops.eval_Stokes_SL_DL_offSurf(ctx, 0, DEL, s, t)
+    sl = hor.ops.eval_Stokes_SL_offSurf(ctx, 0, DEL, s, t)
+    dl = hor.ops.eval_Stokes_DL_offSurf(ctx, 0, DEL, s, t)
+    return fsl, fdl, sl, dl
 
 
 def test_fused_matches_oracle_and_separate_band(ctx, oracle):
