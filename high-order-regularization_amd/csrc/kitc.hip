@@ -27,7 +27,10 @@
 namespace hor {
 
 
-constexpr int kMomChunk = 4096;  // members per moment work item
+// Members per moment work item: enough items to give the moment GEMM about
+// 2048 workgroups, between kMomChunkMin and kMomChunkMax members each.
+constexpr int kMomChunkMin = 256, kMomChunkMax = 4096, kMomWantBlocks = 2048;
+constexpr int kMomReduceBlock = 256;
 
 struct TreePlan {
   Tree T;
@@ -187,20 +190,22 @@ __global__ __launch_bounds__(kBlock) void k_moments(MomArgs A) {
 }
 
 // Sums the member-chunk partials of each far node in chunk order into the
-// proxy-major moments (and their FP32 copy, padded to ncomp32 per proxy).
+// proxy-major moments (and their FP32 copy, padded to ncomp32 per proxy):
+// one thread per (far node, moment value), blockIdx.x the node.
 __global__ void k_moments_reduce(int ncomp, int pflat, const int* node_first, const int* node_nitems,
                                  const double* part, double* mom, float* mom32, int ncomp32) {
   const int f = blockIdx.x;
   const int total = ncomp * pflat;
+  const int e = blockIdx.y * blockDim.x + threadIdx.x;
+  if (e >= total) return;
   const int it0 = node_first[f], nit = node_nitems[f];
-  for (int e = threadIdx.x; e < total; e += blockDim.x) {
-    double s = 0.0;
-    for (int q = 0; q < nit; q++) s += part[(size_t)(it0 + q) * total + e];
-    mom[(size_t)f * total + e] = s;
-    if (mom32) {
-      const int kk = e / ncomp, c = e - kk * ncomp;
-      mom32[((size_t)f * pflat + kk) * ncomp32 + c] = (float)s;
-    }
+  const double* p = part + (size_t)it0 * total + e;
+  double s = 0.0;
+  for (int q = 0; q < nit; q++) s += p[(size_t)q * total];
+  mom[(size_t)f * total + e] = s;
+  if (mom32) {
+    const int kk = e / ncomp, c = e - kk * ncomp;
+    mom32[((size_t)f * pflat + kk) * ncomp32 + c] = (float)s;
   }
 }
 
@@ -498,14 +503,20 @@ static int build_moment_tables(hor_plan* P, const double* sx) {
   }
   std::vector<double> fac(std::max<size_t>(fac_off[TP.nfar_nodes], 1), 0.0);
   double moment_terms = 0;
+  long total_members = 0;
+  for (int q : far_nodes) total_members += T.nodes[q].hi - T.nodes[q].lo + 1;
+  const int yblocks = (np1 * np1 + 63) / 64;   // k_moments workgroups per item
+  const long want_items = std::max(1, kMomWantBlocks / yblocks);
+  const int chunk = (int)std::min<long>(kMomChunkMax, std::max<long>(kMomChunkMin,
+                                        ((total_members + want_items - 1) / want_items + 63) / 64 * 64));
   std::vector<int> mi_far, mi_m0, mi_m1, m_first(TP.nfar_nodes), m_nit(TP.nfar_nodes);
   for (int f = 0; f < TP.nfar_nodes; f++) {
     const int q = far_nodes[f], lo = T.nodes[q].lo, nm = T.nodes[q].hi - lo + 1;
     m_first[f] = (int)mi_far.size();
-    for (int a = 0; a < nm; a += kMomChunk) {
+    for (int a = 0; a < nm; a += chunk) {
       mi_far.push_back(f);
       mi_m0.push_back(a);
-      mi_m1.push_back(std::min(nm, a + kMomChunk));
+      mi_m1.push_back(std::min(nm, a + chunk));
     }
     m_nit[f] = (int)mi_far.size() - m_first[f];
     double* F = &fac[fac_off[f]];
@@ -736,7 +747,9 @@ int tree_plan_eval(hor_plan* P, const double* dens_dev, const double* d0_dev, do
       case 7: hipLaunchKernelGGL(k_moments<7>, mg, dim3(kBlock), 0, s, M); break;
       default: hipLaunchKernelGGL(k_moments<kMomTiles>, mg, dim3(kBlock), 0, s, M); break;
     }
-    hipLaunchKernelGGL(k_moments_reduce, dim3(TP.nfar_nodes), dim3(kBlock), 0, s, TP.ncomp, TP.pflat,
+    const int rblocks = (TP.ncomp * TP.pflat + kMomReduceBlock - 1) / kMomReduceBlock;
+    hipLaunchKernelGGL(k_moments_reduce, dim3(TP.nfar_nodes, rblocks), dim3(kMomReduceBlock), 0, s, TP.ncomp,
+                       TP.pflat,
                        TP.mom_node_first.p, TP.mom_node_nitems.p, TP.mom_part.p, TP.mom.p,
                        mixed ? TP.mom32.p : nullptr, dl ? kNc32Dl : kNc32Sl);
     HOR_HIP_TRY(hipGetLastError());
