@@ -31,6 +31,9 @@ namespace hor {
 // 2048 workgroups, between kMomChunkMin and kMomChunkMax members each.
 constexpr int kMomChunkMin = 256, kMomChunkMax = 4096, kMomWantBlocks = 2048;
 constexpr int kMomReduceBlock = 256;
+// Near sums: split the leaves' sources when fewer than kNearWantItems items
+// (workgroups) would run, into at most kNearMaxSplit chunks.
+constexpr int kNearWantItems = 2048, kNearMaxSplit = 64;
 
 struct TreePlan {
   Tree T;
@@ -44,6 +47,7 @@ struct TreePlan {
   // near work items
   int n_near_items = 0;
   DBuf<int> near_item_node, near_item_off, near_item_cnt, near_tgt, near_slot;
+  DBuf<int> near_item_src;   // per near item: source slots [j0, j1), first subtile, chunk index
   // far work items
   int n_far_items = 0;
   DBuf<int> far_item_node, far_item_off, far_item_cnt, far_tgt, far_slot;
@@ -211,8 +215,8 @@ __global__ void k_moments_reduce(int ncomp, int pflat, const int* node_first, co
 
 struct LeafArgs {
   DirectArgs D;
-  const int *item_node, *item_off, *item_cnt, *tgt, *slot;
-  const int *node_lo, *node_hi, *leaf_sub_off;
+  const int *item_off, *item_cnt, *tgt, *slot;
+  const int* item_src;   // 4 per item: source slots j0, j1, first leaf subtile, chunk (slot offset)
   double* slots;
   int nslot;
 };
@@ -226,13 +230,13 @@ __device__ __forceinline__ double wave_max(double v) {
   return v;
 }
 
-// Leaf direct sums (Call_Ds_*, KITC.cpp:807-1062) for up to 256 near targets.
+// Leaf direct sums (Call_Ds_*, KITC.cpp:807-1062) for up to 256 near targets
+// over one chunk of a leaf's sources.
 template <int KIND, bool MIXED>
 __global__ __launch_bounds__(kBlock) void k_leaf(LeafArgs A) {
   const double* tab = stage_table<KIND>(A.D);   // the workgroup's only barrier
   const int tid = threadIdx.x;
   const int item = blockIdx.x;
-  const int node = A.item_node[item];
   const int off = A.item_off[item], cnt = A.item_cnt[item];
   // a wave without targets leaves at once (the last item of a node's list is
   // partial)
@@ -249,10 +253,10 @@ __global__ __launch_bounds__(kBlock) void k_leaf(LeafArgs A) {
   const bool off_kind = (KIND == HOR_SL_OFF || KIND == HOR_DL_OFF);
   const bool all_far = off_kind && !__any(t.nearp ? 1 : 0);
   double u[3] = {0.0, 0.0, 0.0};
-  const int lo = A.node_lo[node], hi = A.node_hi[node];
-  range_sum<KIND, MIXED, MIXED>(A.D, lo, hi + 1, A.leaf_sub_off[node], wb, all_far, t, tab, u);
+  const int* src = A.item_src + 4 * item;   // this item's chunk of the leaf's sources
+  range_sum<KIND, MIXED, MIXED>(A.D, src[0], src[1], src[2], wb, all_far, t, tab, u);
   if (act) {
-    int s = A.slot[li];
+    int s = A.slot[li] + src[3];
     A.slots[s] = u[0];
     A.slots[A.nslot + s] = u[1];
     A.slots[2 * A.nslot + s] = u[2];
@@ -612,6 +616,42 @@ int tree_plan_create(hor_plan* P, const double* sx, const double* sn, const doub
   std::vector<int> ni_node, ni_off, ni_cnt, n_tg, n_sl, fi_node, fi_off, fi_cnt, f_tg, f_sl;
   items(ntg, nsl, ni_node, ni_off, ni_cnt, n_tg, n_sl);
   items(ftg, fsl, fi_node, fi_off, fi_cnt, f_tg, f_sl);
+  // With few near items (a target set near only a few leaves, as the other
+  // surface's points are) the leaves' sources are split into chunks of whole
+  // 64-source subtiles so that the near sum still has about kNearWantItems
+  // workgroups; each chunk of a (target, leaf) entry gets a slot of its own,
+  // consecutive, summed in order by k_tree_finalize.
+  const int split = std::max(1, std::min(kNearMaxSplit, (kNearWantItems + std::max(1, (int)ni_node.size()) - 1) /
+                                                            std::max(1, (int)ni_node.size())));
+  std::vector<int> nchunk(nn, 1), csub(nn, 1);
+  for (int q = 0; q < nn; q++) {
+    const int nsub = lso[q + 1] - lso[q];
+    const int c = std::max(1, std::min(split, nsub));
+    csub[q] = (nsub + c - 1) / c;               // subtiles per chunk
+    nchunk[q] = (nsub + csub[q] - 1) / csub[q];
+  }
+  std::vector<int> new_start(TP.L.node.size() + 1, 0);   // slot of each list entry
+  for (size_t e = 0; e < TP.L.node.size(); e++)
+    new_start[e + 1] = new_start[e] + (TP.L.far[e] ? 1 : nchunk[TP.L.node[e]]);
+  for (auto& v : n_sl) v = new_start[v];
+  for (auto& v : f_sl) v = new_start[v];
+  std::vector<int> slot_start(nt + 1);
+  for (int o = 0; o <= nt; o++) slot_start[o] = new_start[TP.L.start[o]];
+  TP.nslot = slot_start[nt];
+  std::vector<int> ci_node, ci_off, ci_cnt, ci_src;
+  for (size_t it = 0; it < ni_node.size(); it++) {
+    const int q = ni_node[it];
+    for (int c = 0; c < nchunk[q]; c++) {
+      ci_node.push_back(q);
+      ci_off.push_back(ni_off[it]);
+      ci_cnt.push_back(ni_cnt[it]);
+      const int j0 = lo[q] + c * csub[q] * kWave, j1 = std::min(hi[q] + 1, j0 + csub[q] * kWave);
+      ci_src.insert(ci_src.end(), {j0, j1, lso[q] + c * csub[q], c});
+    }
+  }
+  ni_node.swap(ci_node);
+  ni_off.swap(ci_off);
+  ni_cnt.swap(ci_cnt);
   TP.n_near_items = (int)ni_node.size();
   TP.n_far_items = (int)fi_node.size();
   double near_pairs = 0;
@@ -654,6 +694,7 @@ int tree_plan_create(hor_plan* P, const double* sx, const double* sn, const doub
   HOR_TRY(TP.near_item_node.upload(ni_node, s));
   HOR_TRY(TP.near_item_off.upload(ni_off, s));
   HOR_TRY(TP.near_item_cnt.upload(ni_cnt, s));
+  HOR_TRY(TP.near_item_src.upload(ci_src.empty() ? std::vector<int>{0, 0, 0, 0} : ci_src, s));
   HOR_TRY(TP.near_tgt.upload(n_tg, s));
   HOR_TRY(TP.near_slot.upload(n_sl, s));
   HOR_TRY(TP.far_item_node.upload(fi_node, s));
@@ -662,7 +703,7 @@ int tree_plan_create(hor_plan* P, const double* sx, const double* sn, const doub
   HOR_TRY(TP.far_tgt.upload(f_tg, s));
   HOR_TRY(TP.far_slot.upload(f_sl, s));
   HOR_TRY(TP.mq.alloc((size_t)P->nsp * 16));
-  HOR_TRY(TP.slot_start.upload(TP.L.start, s));
+  HOR_TRY(TP.slot_start.upload(slot_start, s));
   HOR_TRY(TP.slots.alloc(3 * (size_t)std::max(TP.nslot, 1)));
   HOR_TRY(build_moment_tables(P, sx));
   // xi of the treecode off-surface double layer (set by setup_targets for
@@ -759,9 +800,9 @@ int tree_plan_eval(hor_plan* P, const double* dens_dev, const double* d0_dev, do
     LeafArgs A;
     A.D = direct_args(P);
     A.D.sub_box = TP.leaf_box.p;
-    A.item_node = TP.near_item_node.p; A.item_off = TP.near_item_off.p; A.item_cnt = TP.near_item_cnt.p;
+    A.item_off = TP.near_item_off.p; A.item_cnt = TP.near_item_cnt.p;
     A.tgt = TP.near_tgt.p; A.slot = TP.near_slot.p;
-    A.node_lo = TP.node_lo.p; A.node_hi = TP.node_hi.p; A.leaf_sub_off = TP.leaf_sub_off.p;
+    A.item_src = TP.near_item_src.p;
     A.slots = TP.slots.p; A.nslot = TP.nslot;
     switch (P->kind) {
       case HOR_SL_ON: launch_leaf<HOR_SL_ON>(A, mixed, TP.n_near_items, s); break;
