@@ -67,7 +67,7 @@ struct TreePlan {
   // density: the far field reads that plan's far_idx, mom and mom32.
   const TreePlan* mom_owner = nullptr;
   // reduction
-  DBuf<int> slot_start;      // caller order, nt+1
+  DBuf<int> slot_start;      // sorted target order, nt+1
   DBuf<double> slots;        // 3*nslot
 };
 
@@ -427,25 +427,25 @@ __global__ __launch_bounds__(kBlock) void k_far(FarArgs A) {
   }
 }
 
-// Each target sums its slots in visiting order, then v = scale*sum + xi*g0
+// Each target sums its slots in visiting order (one thread per target and
+// component), then v = scale*sum + xi*g0
 // combined into out as the direct finalize does.
 __global__ void k_tree_finalize(int nt, int ntp, int nslot, const int* tperm, const int* slot_start,
                                 const double* slots, double scale, const double* xi, const double* g0,
                                 double* out, const double* base, double coef, int accumulate) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = blockIdx.y;   // component
   if (i >= nt) return;
   int o = tperm[i];
-  int a = slot_start[o], b = slot_start[o + 1];
-  for (int k = 0; k < 3; k++) {
-    double s = 0.0;
-    for (int q = a; q < b; q++) s += slots[(size_t)k * nslot + q];
-    double v = s * scale;
-    if (xi) v += xi[i] * g0[(size_t)k * ntp + i];
-    double w = coef * v;
-    if (accumulate) out[3 * o + k] += w;
-    else if (base) out[3 * o + k] = base[3 * o + k] + w;
-    else out[3 * o + k] = w;
-  }
+  int a = slot_start[i], b = slot_start[i + 1];   // sorted order: a wave reads neighbouring slots
+  double s = 0.0;
+  for (int q = a; q < b; q++) s += slots[(size_t)k * nslot + q];
+  double v = s * scale;
+  if (xi) v += xi[i] * g0[(size_t)k * ntp + i];
+  double w = coef * v;
+  if (accumulate) out[3 * o + k] += w;
+  else if (base) out[3 * o + k] = base[3 * o + k] + w;
+  else out[3 * o + k] = w;
 }
 
 // ---------------------------------------------------------------------------
@@ -630,13 +630,20 @@ int tree_plan_create(hor_plan* P, const double* sx, const double* sn, const doub
     csub[q] = (nsub + c - 1) / c;               // subtiles per chunk
     nchunk[q] = (nsub + csub[q] - 1) / csub[q];
   }
-  std::vector<int> new_start(TP.L.node.size() + 1, 0);   // slot of each list entry
-  for (size_t e = 0; e < TP.L.node.size(); e++)
-    new_start[e + 1] = new_start[e] + (TP.L.far[e] ? 1 : nchunk[TP.L.node[e]]);
+  // slots of each list entry, targets in sorted order (the finalize kernel's
+  // threads), a target's entries in visiting order
+  std::vector<int> new_start(TP.L.node.size(), 0), slot_start(nt + 1, 0);
+  for (int i = 0, next = 0; i < nt; i++) {
+    const int o = P->tperm_h[i];
+    slot_start[i] = next;
+    for (int e = TP.L.start[o]; e < TP.L.start[o + 1]; e++) {
+      new_start[e] = next;
+      next += TP.L.far[e] ? 1 : nchunk[TP.L.node[e]];
+    }
+    slot_start[i + 1] = next;
+  }
   for (auto& v : n_sl) v = new_start[v];
   for (auto& v : f_sl) v = new_start[v];
-  std::vector<int> slot_start(nt + 1);
-  for (int o = 0; o <= nt; o++) slot_start[o] = new_start[TP.L.start[o]];
   TP.nslot = slot_start[nt];
   std::vector<int> ci_node, ci_off, ci_cnt, ci_src;
   for (size_t it = 0; it < ni_node.size(); it++) {
@@ -833,7 +840,7 @@ int tree_plan_eval(hor_plan* P, const double* dens_dev, const double* d0_dev, do
   }
   HOR_HIP_TRY(hipEventRecord(c->ev[3], s));
   // 4. per-target reduction
-  hipLaunchKernelGGL(k_tree_finalize, dim3((P->nt + kBlock - 1) / kBlock), dim3(kBlock), 0, s, P->nt, P->ntp,
+  hipLaunchKernelGGL(k_tree_finalize, dim3((P->nt + kBlock - 1) / kBlock, 3), dim3(kBlock), 0, s, P->nt, P->ntp,
                      TP.nslot, P->tperm.p, TP.slot_start.p, TP.slots.p,
                      dl ? -6.0 / (8.0 * kPi) : 1.0 / (8.0 * kPi),   // KITC.cpp:1395, 1612
                      dl ? P->xi.p : nullptr, dl ? P->tsub2.p : nullptr, out_dev, base, coef, accumulate);
