@@ -2,6 +2,7 @@
 // interaction list.  Both depend on geometry only, so they are built once per
 // plan (the reference rebuilds the tree on every operator call,
 // KITC.cpp:1435-1436).
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -131,6 +132,24 @@ void tree_build(Tree& T, int ns, const double* sx, const double ctr[3], int P, i
   root.parent = -1;
   T.nodes.push_back(root);
   B.build(0);
+  // The sources of each childless node in Morton order of their positions
+  // (the split keeps the input order within an octant): the near sums'
+  // 64-source subtiles then have tight boxes.  Node ranges and members are
+  // unchanged.
+  for (const TreeNode& nd : T.nodes) {
+    const int n = nd.hi - nd.lo + 1;
+    if (nd.nchild || n < 2) continue;
+    const double lo[3] = {nd.iv[0], nd.iv[2], nd.iv[4]};
+    const double ext = std::max(nd.iv[1] - nd.iv[0], std::max(nd.iv[3] - nd.iv[2], nd.iv[5] - nd.iv[4]));
+    std::vector<std::pair<uint64_t, int>> kp(n);
+    for (int k = 0; k < n; k++) {
+      const int o = T.perm[nd.lo + k];
+      kp[k] = {morton_key(sx + 3 * (size_t)o, lo, ext), o};
+    }
+    std::stable_sort(kp.begin(), kp.end(),
+                     [](const std::pair<uint64_t, int>& a, const std::pair<uint64_t, int>& b) { return a.first < b.first; });
+    for (int k = 0; k < n; k++) T.perm[nd.lo + k] = kp[k].second;
+  }
   // Cluster_Chev_Points (KITC.cpp:780-804): Chebyshev-Lobatto nodes per box.
   int np1 = P + 1;
   std::vector<double> t(np1);
