@@ -320,9 +320,15 @@ static int plan_create_direct(hor_plan* P, const double* sx, const double* sn, c
   HOR_TRY(setup_sources(P, spatial_order(P->ns, sx, nullptr), sx, sn, sa));
   HOR_TRY(setup_targets(P, tx, tb, tx0, tn0, sx));
   P->nsub = (P->ns + kWave - 1) / kWave;
+  // Source chunks (grid.y): enough workgroups for ~40 rounds of the chip's
+  // 2048 resident ones, so the last round's partial fill costs little (one
+  // chunk at 1.3M targets leaves 2.5 rounds), and at least 8 subtiles (512
+  // sources) per chunk, so the partial-sum traffic stays small against the
+  // pair work.  The partial buffer is then <= ~82k * 256 * 6 doubles.
+  constexpr int kWantBlocks = 40 * 2048, kMinSubsPerChunk = 8, kMaxChunks = 4096;
   int nblk = P->ntp / kBlock;
-  int want = std::max(1, (2048 + nblk - 1) / nblk);  // >= 8 workgroups per CU
-  int nchunk = std::max(1, std::min(P->nsub, want));
+  int want = std::max(1, (kWantBlocks + nblk - 1) / nblk);
+  int nchunk = std::min({want, std::max(1, P->nsub / kMinSubsPerChunk), kMaxChunks});
   P->subs_per_chunk = (P->nsub + nchunk - 1) / nchunk;
   P->nchunk = (P->nsub + P->subs_per_chunk - 1) / P->subs_per_chunk;
   HOR_TRY(P->partial.alloc((size_t)P->nchunk * (P->kind == HOR_SLDL_OFF ? 6 : 3) * P->ntp));
