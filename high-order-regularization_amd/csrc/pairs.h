@@ -75,16 +75,21 @@ struct Tgt {
 
 // ---- SL pair: plain kernel with subtraction a [f - (f0.n0) n] (fast path):
 //   u += a f~ / r + dx (a f~ . dx) / r^3  =  (a f~ + dx t) / r,  t = (a f~ . dx) / r^2
+// with f~ . dx = f.dx - (f0.n0)(n.dx) and a f~ + dx t = (a f + dx t) - (f0.n0) a n:
+// every FMA then reads one source value, which the wave holds in scalar
+// registers, and f~ is never formed (a VALU instruction reads at most one
+// scalar register, so f - (f0.n0) n would first copy f to vector registers).
 __device__ __forceinline__ void sl_plain(const SrcRec& s, double px, double py, double pz, double sub,
                                          double u[3]) {
   double dx = px - s.x, dy = py - s.y, dz = pz - s.z;
   double r2 = dx * dx + dy * dy + dz * dz;
   double ri = rsqrt64(r2);
-  double fx = s.d0 - sub * s.nx, fy = s.d1 - sub * s.ny, fz = s.d2 - sub * s.nz;
-  double t = (fx * dx + fy * dy + fz * dz) * (ri * ri);
-  u[0] += (fx + dx * t) * ri;
-  u[1] += (fy + dy * t) * ri;
-  u[2] += (fz + dz * t) * ri;
+  double fdx = s.d0 * dx + s.d1 * dy + s.d2 * dz;
+  double ndx = s.nx * dx + s.ny * dy + s.nz * dz;
+  double t = (fdx - sub * ndx) * (ri * ri);
+  u[0] += ((s.d0 + dx * t) - sub * s.nx) * ri;
+  u[1] += ((s.d1 + dy * t) - sub * s.ny) * ri;
+  u[2] += ((s.d2 + dz * t) - sub * s.nz) * ri;
 }
 
 // The same in FP32 for a given separation d (FP32 partial sums).
@@ -234,19 +239,19 @@ __device__ __forceinline__ void dl_off_full(const SrcRec& s, const Tgt& t, doubl
 // ---- Stokeslet + stresslet at one off-surface target (HOR_SLDL_OFF): the
 // pair's separation, r^2 and 1/r are computed once; the single-layer terms
 // are sl_plain's / sl_full's and the double-layer terms dl_plain's /
-// dl_off_full's, expression for expression.
+// dl_off_full's, expression for expression; n.dx serves both.
 __device__ __forceinline__ void sldl_plain(const SrcRecV& s, const Tgt& t, double us[3], double ud[3]) {
   double dx = t.px - s.x, dy = t.py - s.y, dz = t.pz - s.z;
   double r2 = dx * dx + dy * dy + dz * dz;
   double ri = rsqrt64(r2);
-  double fx = s.f0 - t.sub * s.nx, fy = s.f1 - t.sub * s.ny, fz = s.f2 - t.sub * s.nz;
-  double tq = (fx * dx + fy * dy + fz * dz) * (ri * ri);
-  us[0] += (fx + dx * tq) * ri;
-  us[1] += (fy + dy * tq) * ri;
-  us[2] += (fz + dz * tq) * ri;
+  double fdx = s.f0 * dx + s.f1 * dy + s.f2 * dz;
+  double ndx = s.nx * dx + s.ny * dy + s.nz * dz;
+  double tq = (fdx - t.sub * ndx) * (ri * ri);
+  us[0] += ((s.f0 + dx * tq) - t.sub * s.nx) * ri;
+  us[1] += ((s.f1 + dy * tq) - t.sub * s.ny) * ri;
+  us[2] += ((s.f2 + dz * tq) - t.sub * s.nz) * ri;
   double ri2 = ri * ri;
   double gdx = (s.g0 - t.g0x) * dx + (s.g1 - t.g0y) * dy + (s.g2 - t.g0z) * dz;
-  double ndx = s.nx * dx + s.ny * dy + s.nz * dz;
   double k = gdx * ndx * (ri2 * ri2 * ri);
   ud[0] += dx * k;
   ud[1] += dy * k;
@@ -463,14 +468,14 @@ __device__ __forceinline__ void sldl_plain32(const SrcRecV32& s, float dx, float
                                              float ws[3], float wd[3]) {
   float r2 = dx * dx + dy * dy + dz * dz;
   float ri = rsqrt32(r2);
-  float fx = s.f0 - v.sub * s.nx, fy = s.f1 - v.sub * s.ny, fz = s.f2 - v.sub * s.nz;
-  float tq = (fx * dx + fy * dy + fz * dz) * (ri * ri);
-  ws[0] += (fx + dx * tq) * ri;
-  ws[1] += (fy + dy * tq) * ri;
-  ws[2] += (fz + dz * tq) * ri;
+  float fdx = s.f0 * dx + s.f1 * dy + s.f2 * dz;
+  float ndx = s.nx * dx + s.ny * dy + s.nz * dz;
+  float tq = (fdx - v.sub * ndx) * (ri * ri);
+  ws[0] += ((s.f0 + dx * tq) - v.sub * s.nx) * ri;
+  ws[1] += ((s.f1 + dy * tq) - v.sub * s.ny) * ri;
+  ws[2] += ((s.f2 + dz * tq) - v.sub * s.nz) * ri;
   float ri2 = ri * ri;
   float gdx = (s.g0 - v.g0x) * dx + (s.g1 - v.g0y) * dy + (s.g2 - v.g0z) * dz;
-  float ndx = s.nx * dx + s.ny * dy + s.nz * dz;
   float k = gdx * ndx * (ri2 * ri2 * ri);
   wd[0] += dx * k;
   wd[1] += dy * k;
