@@ -561,14 +561,18 @@ __device__ __forceinline__ void dl_on_reg32(const SrcRec32& s, float dx, float d
 // ---- Pressure pair (stokes_pressure_7ord, extend_to_grid/stokes.cpp:134-203).
 // The reference's summand [-(f.n - f0.n0)(n.dx) - (n x f - n0 x f0).(n x dx)] A
 // equals dx.w with w = -(a f) + (f0.n0)(a n) + (n0 x f0) x (a n) for a unit
-// normal n (Binet-Cauchy and the triple product), so with the record's a f
-// and a n every pair costs one weight vector and one dot product.  Tgt.sub
-// holds f0.n0 and Tgt.g0 holds n0 x f0.
+// normal n (Binet-Cauchy and the triple product).  Tgt.sub holds f0.n0 and
+// Tgt.g0 holds n0 x f0.  dx.w is evaluated as
+//   (f0.n0)(a n . dx) - a f . dx + a n . (dx x (n0 x f0)):
+// every product then reads at most one source value (the wave holds the
+// record in scalar registers and a VALU instruction reads at most one), so
+// no record value is first copied to vector registers.
 __device__ __forceinline__ double p_numer(const SrcRec& s, const Tgt& t, double dx, double dy, double dz) {
-  const double wx = t.sub * s.nx - s.d0 + (t.g0y * s.nz - t.g0z * s.ny);
-  const double wy = t.sub * s.ny - s.d1 + (t.g0z * s.nx - t.g0x * s.nz);
-  const double wz = t.sub * s.nz - s.d2 + (t.g0x * s.ny - t.g0y * s.nx);
-  return wx * dx + wy * dy + wz * dz;
+  const double nd = s.nx * dx + s.ny * dy + s.nz * dz;
+  const double fd = s.d0 * dx + s.d1 * dy + s.d2 * dz;
+  const double cx = dy * t.g0z - dz * t.g0y, cy = dz * t.g0x - dx * t.g0z, cz = dx * t.g0y - dy * t.g0x;
+  const double nc = s.nx * cx + s.ny * cy + s.nz * cz;
+  return (t.sub * nd - fd) + nc;
 }
 
 // Plain kernel H2 = 1/r^3 (Lapl_DL_7ord beyond 8 delta).
