@@ -24,6 +24,15 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path[:0] = [os.path.join(ROOT, "high-order-regularization_amd"), os.path.join(ROOT, "oracle")]
 
+# OpenMP placement of the CPU baseline legs (read by libgomp when the first
+# OpenMP library loads, so before any import of the package): threads packed
+# onto cores.  OMP_NUM_THREADS is left to the environment (the process's CPU
+# share); host_cpu() records what the threads ran on.
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
+
+from hor import solver as TS  # noqa: E402
+
 H = 1.0 / 64.0
 BAND = 0.3
 ABC = (1.0, 0.8, 0.6)
@@ -41,6 +50,35 @@ OPS = {"sl_plain": 30, "sl_reg": 61, "dl_plain": 30, "dl_reg": 127, "p_plain": 3
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def host_cpu():
+    """The host CPUs of the CPU baseline legs: lscpu's model, sockets, cores
+    per socket and threads per core, the CPUs this process may run on, and the
+    OpenMP settings (threads = OMP_NUM_THREADS, else the CPUs available)."""
+    info = {}
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k in ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core", "CPU(s)"):
+                info[k] = v
+    except (OSError, ValueError):
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    try:
+        physical = int(info["Socket(s)"]) * int(info["Core(s) per socket"])
+    except (KeyError, ValueError):
+        physical = None
+    threads = int(os.environ.get("OMP_NUM_THREADS", avail))
+    return {"threads": threads, "cpus_available": avail, "physical_cores_machine": physical,
+            "lscpu": info, "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS"),
+            "OMP_PROC_BIND": os.environ.get("OMP_PROC_BIND"), "OMP_PLACES": os.environ.get("OMP_PLACES")}
 
 
 def build_workload(h=H, band=BAND):
@@ -79,10 +117,8 @@ def cpu_baseline(s, t, DEL, target_seconds=15.0):
     (compiled from its sources into oracle/_ref) on a bounded target sample,
     all host cores; falls back to the oracle restatement when the reference
     build is absent."""
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except AttributeError:
-        cores = os.cpu_count() or 1
+    hc = host_cpu()
+    cores = hc["threads"]
     kind = "reference"
     try:
         from pyoracle import Ref2
@@ -111,17 +147,16 @@ def cpu_baseline(s, t, DEL, target_seconds=15.0):
     dt = run(tt)
     pairs = 2.0 * n * s["n"]
     return {"value": pairs / dt, "unit": "pair interactions/s", "cores": cores, "kind": kind,
-            "sample": f"{n} of {t['n']} band targets x {s['n']} sources, SL_OFF + DL_OFF, {dt:.1f}s"}
+            "sample": f"{n} of {t['n']} band targets x {s['n']} sources, SL_OFF + DL_OFF, {dt:.1f}s",
+            "host": hc}
 
 
 def pressure_cpu_baseline(s, t, DEL, target_seconds=6.0):
     """The reference's stokes_pressure_7ord (compiled into oracle/_ref) on a
     bounded target sample, all host cores; the oracle restatement when the
     reference build is absent."""
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except AttributeError:
-        cores = os.cpu_count() or 1
+    hc = host_cpu()
+    cores = hc["threads"]
     try:
         from pyoracle import RefGrid
         impl, kind = RefGrid().pressure_7ord, "reference"
@@ -148,34 +183,6 @@ def pressure_cpu_baseline(s, t, DEL, target_seconds=6.0):
             "sample": f"{n} of {t['n']} band targets x {s['n']} sources, stokes_pressure_7ord, {dt:.1f}s"}
 
 
-def two_sphere_problem(h, gap, seed=SEED, ctx=None):
-    """Two spheres of radius 0.5, centres (0,0,+-(1+gap)/2), the surface
-    dicts of the two-surface solver (nearest data of each surface's points
-    on the other surface) and two right-hand sides:
-      rigid:   seeded rigid-body data b = U_k + W_k x (x - c_k), U, W ~ U(-1,1);
-      forcing: the reference's b = -2/3 SL[f] with its surface force f
-               (main.cpp:199-207, surface.cpp:530-559).
-    Quadrature and nearest data are built on the GPU of ctx when given, else
-    on the host."""
-    from hor import geometry as G
-    DEL = G.delta(h)
-    zc = 0.5 + gap / 2
-    bodies = [G.Body((0.5,) * 3, (0.0, 0.0, -zc)), G.Body((0.5,) * 3, (0.0, 0.0, zc))]
-    surfs = [b.surface(h, ctx=ctx) for b in bodies]
-    pd = []
-    for k in range(2):
-        t = bodies[1 - k].nearest(h, DEL, surfs[1 - k], surfs[k]["x"], with_force=True, ctx=ctx)
-        pd.append(dict(surfs[k], ctr=bodies[k].ctr, nrst=t["nrst"], x0=t["x0"], n0=t["n0"], b=t["b"],
-                       i123=t["i123"], f=bodies[k].force(surfs[k]["x"], surfs[k]["nrml"]), f0=t["f0"]))
-    rng = np.random.default_rng(seed)
-    rigid = []
-    for k in range(2):
-        U, Wv = rng.uniform(-1, 1, 3), rng.uniform(-1, 1, 3)
-        x = pd[k]["x"].reshape(-1, 3)
-        rigid.append((U + np.cross(Wv, x - bodies[k].ctr)).ravel())
-    return DEL, bodies, pd, np.concatenate(rigid)
-
-
 def geometry_setup(ctx, h, gap, pd_gpu, gpu_s, sample=256):
     """Geometry set-up on the GPU against the host: the same quadrature and
     nearest data built by the host functions (time, and whether every
@@ -184,7 +191,7 @@ def geometry_setup(ctx, h, gap, pd_gpu, gpu_s, sample=256):
     point per target) timed on a sample of near targets and scaled to all."""
     from hor import geometry as G
     t0 = time.perf_counter()
-    _, _, pd_host, _ = two_sphere_problem(h, gap)
+    _, _, pd_host, _ = TS.two_sphere_problem(h, gap)
     host_s = time.perf_counter() - t0
     same = all(np.array_equal(pd_gpu[k][key], pd_host[k][key]) for k in range(2)
                for key in ("x", "nrml", "plane", "nrst", "i123", "x0", "n0", "b", "f0"))
@@ -239,16 +246,16 @@ def cpu_matvec(h, DEL, pd, P, N0, theta, u, reps=1):
 def gmres_workload(h=1.0 / 128.0, gap=0.1, P=8, N0=4000, theta=0.6, tol=1e-8, seed=SEED,
                    direct_check=False, cpu=False, cpu_reps=1, geometry=False):
     """Second-kind double-layer equation A u = u - 2/3 DL u = b on two spheres
-    (two_sphere_problem), treecode matvec, GMRES(15) to relative residual tol,
+    (hor.solver.two_sphere_problem), treecode matvec, GMRES(15) to relative residual tol,
     FP64 and mixed-precision treecodes, both right-hand sides.  Every solution's
     residual is rechecked with the FP64 treecode operator (and, with
     direct_check, with the FP64 direct-sum operator)."""
     import hor
     from hor.solver import Solver
     ctx = hor.Context(0)
-    two_sphere_problem(1.0 / 16.0, gap, seed, ctx=ctx)  # first GPU geometry call (code object load)
+    TS.two_sphere_problem(1.0 / 16.0, gap, seed, ctx=ctx)  # first GPU geometry call (code object load)
     t0 = time.perf_counter()
-    DEL, bodies, pd, b_rigid = two_sphere_problem(h, gap, seed, ctx=ctx)
+    DEL, bodies, pd, b_rigid = TS.two_sphere_problem(h, gap, seed, ctx=ctx)
     geo_s = time.perf_counter() - t0
     tp = hor.TreeParams(P, N0, theta)
     n = b_rigid.size
@@ -276,10 +283,7 @@ def gmres_workload(h=1.0 / 128.0, gap=0.1, P=8, N0=4000, theta=0.6, tol=1e-8, se
                               evaluated_interactions_per_s=(st["near_pairs"] + st["proxy_pairs"] +
                                                             st["moment_terms"]) / mv_s)
     if cpu:
-        try:
-            cores = len(os.sched_getaffinity(0))
-        except AttributeError:
-            cores = os.cpu_count() or 1
+        cores = host_cpu()["threads"]
         log(f"CPU treecode matvec (oracle restatement, {cores} threads)")
         t_cpu, t_rhs, y_cpu = cpu_matvec(h, DEL, pd, P, N0, theta, b_force, reps=cpu_reps)
         y_gpu = S64.matvec(b_force)
@@ -309,10 +313,19 @@ def gmres_workload(h=1.0 / 128.0, gap=0.1, P=8, N0=4000, theta=0.6, tol=1e-8, se
             if M is None:
                 x, it, hist, est = S.gmres(b, m=15, tol=tol, maxit=300)
                 info = {"iterations": it, "relres_estimate": est}
+                if S is S64:
+                    # the FP64 solve stops on its estimate; its FP64 residual check is part of the
+                    # timed solve, as the mixed and learned solves end on one inside theirs
+                    S64.matvec(x)
+                    info["final_fp64_check_in_time"] = True
             else:
                 x, st = MP.solve(S, b, net=M, m=15, tol=tol, maxit=300)
-                info = {"iterations": st["learned_iters"] + st["plain_iters"], "learned_iterations": st["learned_iters"],
-                        "fallback_plain_iterations": st["plain_iters"], "relres_estimate": st["relres"]}
+                info = {"iterations": st["iterations"], "learned_steps": st["learned_steps"],
+                        "plain_steps": st["plain_steps"], "gate_cycles_on": st["cycles_gated_on"],
+                        "gate_cycles_off": st["cycles_gated_off"], "monitor_stops": st["monitor_stops"],
+                        "gate_misfit_identity": st["gate_misfit_identity"],
+                        "gate_misfit_network": st["gate_misfit_network"],
+                        "nonfinite_outputs": st["nonfinite_outputs"], "relres_fp64": st["relres"]}
             dt = time.perf_counter() - t0
             xs[name] = x
             r = b - S64.matvec(x)
@@ -337,6 +350,35 @@ def gmres_workload(h=1.0 / 128.0, gap=0.1, P=8, N0=4000, theta=0.6, tol=1e-8, se
         S.close()
     if S_dir is not None:
         S_dir.close()
+    ctx.close()
+    return out
+
+
+def matvec_workload(h=1.0 / 256.0, gap=0.1, P=10, N0=8000, theta=0.6, reps=3, seed=SEED):
+    """Only the two-sphere treecode matvecs (the GMRES operator): reps FP64
+    and reps mixed-precision matvecs of the seeded rigid-body vector, after one
+    warm-up each.  The short run that the profiler passes measure."""
+    import hor
+    from hor.solver import Solver
+    ctx = hor.Context(0)
+    DEL, bodies, pd, b = TS.two_sphere_problem(h, gap, seed, ctx=ctx)
+    tp = hor.TreeParams(P, N0, theta)
+    out = {"workload": f"two spheres r=0.5, gap {gap}, h=1/{round(1 / h)}, treecode P={P} N0={N0} theta={theta}: "
+                       f"{reps} FP64 and {reps} mixed matvecs", "unknowns": int(b.size)}
+    for name, prec in (("fp64", hor.FP64), ("mixed", hor.MIXED)):
+        S = Solver(ctx, h, DEL, pd[0], pd[1], method=hor.TREE, precision=prec, tree1=tp, tree2=tp)
+        ud, yd = ctx.alloc(b.nbytes), ctx.alloc(b.nbytes)
+        ctx.upload(ud, b)
+        S.matvec_dev(ud, yd)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            S.matvec_dev(ud, yd)
+        out[f"{name}_matvec_ms"] = 1e3 * (time.perf_counter() - t0) / reps
+        out[f"{name}_work"] = S.stats()
+        log(f"{name}: {out[f'{name}_matvec_ms']:.1f} ms per matvec")
+        ctx.free(ud)
+        ctx.free(yd)
+        S.close()
     ctx.close()
     return out
 
@@ -371,10 +413,13 @@ def main():
     ap.add_argument("--no-gmres", action="store_true", help="skip the two-sphere GMRES workload")
     ap.add_argument("--no-grid", action="store_true", help="skip the grid-program leg")
     ap.add_argument("--no-h256", action="store_true", help="skip the h=1/256 two-sphere GMRES workload")
-    ap.add_argument("--cpu-h256", action="store_true",
-                    help="also time two CPU treecode matvecs and the RHS at h=1/256 (minutes of CPU time)")
+    ap.add_argument("--no-cpu-h256", action="store_true",
+                    help="skip the CPU treecode matvecs and RHS at h=1/256 (about 80 s of CPU time)")
     ap.add_argument("--gmres-only", type=int, default=0, metavar="N",
                     help="run only the two-sphere GMRES workload at h=1/N (128 or 256) and print its JSON")
+    ap.add_argument("--matvec-only", type=int, default=0, metavar="N",
+                    help="run only FP64 and mixed two-sphere treecode matvecs at h=1/N (256: P=10, N0=8000; "
+                         "128: P=8, N0=4000) and print their JSON (the profiler runs)")
     ap.add_argument("--h", type=float, default=H)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -383,11 +428,15 @@ def main():
         sys.exit(2)
     import hor
 
+    if args.matvec_only:
+        P, N0 = (10, 8000) if args.matvec_only >= 256 else (8, 4000)
+        print(json.dumps(matvec_workload(h=1.0 / args.matvec_only, P=P, N0=N0)), flush=True)
+        return
     if args.gmres_only:
         if args.gmres_only == 128:
             print(json.dumps(gmres_workload()), flush=True)
         else:
-            print(json.dumps(gmres_workload(h=1.0 / args.gmres_only, P=10, N0=8000, cpu=args.cpu_h256,
+            print(json.dumps(gmres_workload(h=1.0 / args.gmres_only, P=10, N0=8000, cpu=not args.no_cpu_h256,
                                             cpu_reps=2, geometry=True)), flush=True)
         return
     body, s, t, DEL = build_workload(h=args.h)
@@ -473,7 +522,7 @@ def main():
     kdl_ms = float(np.mean(kdl))
     ksl_ms = float(np.mean(ksl))
     achieved = (sl_flops + dl_flops) / (kv_ms * 1e-3) / 1e12
-    roof = {"bound": "mfma", "compute": "fp64 vector ALU (peak equals the FP64 matrix peak)",
+    roof = {"bound": "valu", "compute": "FP64 vector ALU issue (peak 78.6 TFLOP/s, equal to the FP64 matrix peak)",
             "kernel": "k_direct_sldl", "achieved": achieved, "peak": PEAK_FP64 / 1e12,
             "unit": "TFLOP/s", "frac": achieved * 1e12 / PEAK_FP64, "traffic": profile_traffic(),
             "avg_kernel_ms": kv_ms, "flop_per_launch": sl_flops + dl_flops,
@@ -537,7 +586,8 @@ def main():
         if not args.no_h256:
             log("two-sphere GMRES workload (h=1/256, treecode P=10 N0=8000)")
             gm["h256"] = gmres_workload(h=1.0 / 256.0, P=10, N0=8000, direct_check=True,
-                                        cpu=args.cpu_h256, cpu_reps=2, geometry=not args.no_cpu)
+                                        cpu=not (args.no_cpu or args.no_cpu_h256), cpu_reps=2,
+                                        geometry=not args.no_cpu)
     line = {
         "metric": "pair interactions/s",
         "value": value,

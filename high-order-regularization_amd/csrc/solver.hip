@@ -96,8 +96,11 @@ struct hor_solver {
   hor_plan* off64[2] = {nullptr, nullptr};
   DBuf<int> nrst[2], st[2];
   DBuf<double> co[2], g0[2];
-  DBuf<double> vin, vout, V, Z, w, part, bvec, xvec, r;
+  DBuf<double> vin, vout, V, Z, w, part, bvec, xvec, r, gt;
   std::vector<double> part_h;
+  // learned-preconditioner policy of hor_solver_fgmres and the last solve's record
+  hor_precond_policy policy{0, 1, 0.5};
+  hor_fgmres_info info{};
   ~hor_solver() {
     for (int k = 0; k < 2; k++) {
       hor_plan_destroy(on[k]);
@@ -355,18 +358,40 @@ extern "C" int hor_solver_eval_sl(hor_solver* S, const double* f1, const double*
 }
 
 // Restarted flexible GMRES(m) with MGS Arnoldi and Givens rotations
-// (main.cpp:210-416), device-resident.  Arnoldi applies A_arn to z_j = M(v_j)
-// (z_j = v_j without a preconditioner); each cycle restarts from the true
+// (main.cpp:210-416), device-resident.  Arnoldi applies A_arn to z_j, where
+// z_j = v_j (a plain step) or z_j = M(v_j) (a learned step); FGMRES allows the
+// choice to change from step to step.  Each cycle restarts from the true
 // residual b - A_true x (the reference restarts from b, main.cpp:286).  With
 // recheck, a cycle that meets tol by its estimate is confirmed with A_true and
-// continues if the true residual is above tol; *relres is then the true
-// residual.  Without it the solve stops on the estimate, as the reference does.
+// continues if the true residual is above tol, and a solve that reaches maxit
+// also ends on the true residual: *relres is then always the true residual.
+// Without it the solve stops on the estimate, as the reference does.
 // Optionally logs the pairs (z_j, A z_j) into device buffers.
+//
+// Learned steps (a network M is given and the policy does not force plain):
+//   - gate: the first step of every cycle is plain.  Its pair (v_0, w = A v_0)
+//     tells how well M inverts A on this cycle's data: with the misfit of an
+//     approximate inverse X taken up to scale, e(X)^2 = 1 - (X(w).v_0)^2 /
+//     (|X(w)|^2 |v_0|^2), the rest of the cycle uses M only if
+//     e(M) < ratio * e(I).  A cycle whose gate rejects M is plain GMRES.
+//   - progress monitor: the plain first step reduces the residual estimate by
+//     rho_0 = e(I); a learned step that reduces it by less (est_j / est_{j-1}
+//     > rho_0) ends the network's use for the rest of the cycle.
+//   - every network output is screened: a non-finite z_j is replaced by v_j
+//     and M is not used again in this solve.
 struct PrecondOp {
   hor_mpnn* net = nullptr;
+  bool gate = true;
+  double ratio = 1.0;
 };
 
 static int apply_precond(hor_solver* S, const PrecondOp& M, const double* v, double* z);
+
+static bool all_finite_host(const double* a, size_t n) {
+  for (size_t i = 0; i < n; i++)
+    if (!std::isfinite(a[i])) return false;
+  return true;
+}
 
 static int gmres_core(hor_solver* S, bool flexible, const PrecondOp& M, bool mixed_arnoldi, bool recheck,
                       const double* b_host, double* x_host, int m, double tol, int maxit, int* iters,
@@ -375,8 +400,12 @@ static int gmres_core(hor_solver* S, bool flexible, const PrecondOp& M, bool mix
   hipStream_t s = c->stream;
   HOR_HIP_TRY(hipSetDevice(c->device));
   const int n = S->n;
+  S->info = hor_fgmres_info{};
+  S->info.gate_misfit_identity = S->info.gate_misfit_network = -1.0;
+  if (!all_finite_host(b_host, (size_t)n)) return set_error(HOR_E_NAN, "GMRES: non-finite right-hand side");
   HOR_TRY(S->V.alloc((size_t)n * (m + 1)));
   if (flexible) HOR_TRY(S->Z.alloc((size_t)n * m));
+  if (flexible && M.net) HOR_TRY(S->gt.alloc(n));
   HOR_TRY(S->w.alloc(n));
   HOR_TRY(S->bvec.alloc(n));
   HOR_TRY(S->xvec.alloc(n));
@@ -395,10 +424,11 @@ static int gmres_core(hor_solver* S, bool flexible, const PrecondOp& M, bool mix
   int iter = 0, logged = 0;
   double error = 1.0;
   bool have_x = false;
+  bool net_alive = flexible && M.net != nullptr;   // cleared for good by a non-finite output
   while (true) {
     // r = b - A_true x (x = 0 on the first cycle: r = b)
     if (have_x) {
-      if (iter >= maxit) break;
+      if (iter >= maxit && !recheck) break;
       HOR_TRY(matvec64_dev(S, S->xvec.p, S->r.p));
       hipLaunchKernelGGL(k_sub, gr, bl, 0, s, n, S->bvec.p, S->r.p, Vp(0));
     } else {
@@ -407,6 +437,7 @@ static int gmres_core(hor_solver* S, bool flexible, const PrecondOp& M, bool mix
     double rn2;
     HOR_TRY(dot_dev(S, Vp(0), Vp(0), &rn2));
     const double r_norm = std::sqrt(rn2);
+    if (!std::isfinite(r_norm)) return set_error(HOR_E_NAN, "GMRES: non-finite residual after %d iterations", iter);
     error = r_norm;
     if (b_norm > 1.e-8) error /= b_norm;
     if (error <= tol || r_norm == 0.0 || iter >= maxit) break;
@@ -415,11 +446,54 @@ static int gmres_core(hor_solver* S, bool flexible, const PrecondOp& M, bool mix
     g[0] = r_norm;
     int j = -1;
     double est = error;
+    // this cycle's steps use the network: decided by the gate after step 0
+    bool use_net = net_alive && !M.gate;
+    double rho0 = -1.0;  // residual reduction of the cycle's plain first step
     while ((j < m - 1) && (est > tol) && (iter < maxit)) {
       j = j + 1;
       iter = iter + 1;
-      if (flexible) HOR_TRY(apply_precond(S, M, Vp(j), Zp(j)));
+      const double est_prev = est;
+      bool learned = false;
+      if (flexible) {
+        if (use_net && net_alive) {
+          HOR_TRY(apply_precond(S, M, Vp(j), Zp(j)));
+          double zz;
+          HOR_TRY(dot_dev(S, Zp(j), Zp(j), &zz));
+          if (!std::isfinite(zz)) {  // screen: fall back to the plain step
+            S->info.nonfinite_outputs++;
+            net_alive = false;
+          }
+        }
+        if (use_net && net_alive) {
+          S->info.learned_steps++;
+          learned = true;
+        } else {
+          HOR_HIP_TRY(hipMemcpyAsync(Zp(j), Vp(j), sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+          S->info.plain_steps++;
+        }
+      }
       HOR_TRY(A_arn(Zp(j), Vp(j + 1)));
+      if (j == 0 && net_alive && M.gate) {
+        // gate on the pair (v_0, w = A v_0), |v_0| = 1
+        HOR_TRY(apply_precond(S, M, Vp(1), S->gt.p));
+        double wv, ww, tv, tt;
+        HOR_TRY(dot_dev(S, Vp(1), Vp(0), &wv));
+        HOR_TRY(dot_dev(S, Vp(1), Vp(1), &ww));
+        HOR_TRY(dot_dev(S, S->gt.p, Vp(0), &tv));
+        HOR_TRY(dot_dev(S, S->gt.p, S->gt.p, &tt));
+        if (!std::isfinite(tt) || !std::isfinite(tv)) {
+          S->info.nonfinite_outputs++;
+          net_alive = false;
+        } else {
+          const double eI = std::sqrt(std::max(0.0, 1.0 - (ww > 0.0 ? wv * wv / ww : 0.0)));
+          const double eM = std::sqrt(std::max(0.0, 1.0 - (tt > 0.0 ? tv * tv / tt : 0.0)));
+          S->info.gate_misfit_identity = eI;
+          S->info.gate_misfit_network = eM;
+          use_net = eM < M.ratio * eI;
+          if (use_net) S->info.cycles_gated_on++;
+          else S->info.cycles_gated_off++;
+        }
+      }
       if (log_z && logged < log_max) {
         HOR_HIP_TRY(hipMemcpyAsync(log_z + (size_t)logged * n, Zp(j), sizeof(double) * n,
                                    hipMemcpyDeviceToDevice, s));
@@ -453,8 +527,15 @@ static int gmres_core(hor_solver* S, bool flexible, const PrecondOp& M, bool mix
       g[j] = cs[j] * g[j];
       est = std::fabs(g[j + 1]);
       if (b_norm > 1.e-8) est /= b_norm;
+      if (!std::isfinite(est)) return set_error(HOR_E_NAN, "GMRES: non-finite Arnoldi step at iteration %d", iter);
       if (hist) hist[iter - 1] = est;
+      if (j == 0 && !learned && est_prev > 0.0) rho0 = est / est_prev;
+      if (learned && M.gate && rho0 > 0.0 && est > rho0 * est_prev) {  // progress monitor
+        use_net = false;
+        S->info.monitor_stops++;
+      }
     }
+    if (j < 0) break;  // no step was taken (not reached: error > tol and iter < maxit here)
     // x += Z y with H y = g (upper triangular)
     for (int i = 0; i <= j; i++) y[i] = g[i];
     y[j] = y[j] / Hh(j, j);
@@ -474,6 +555,8 @@ static int gmres_core(hor_solver* S, bool flexible, const PrecondOp& M, bool mix
   if (iters) *iters = iter;
   if (relres) *relres = error;
   if (n_logged) *n_logged = logged;
+  S->info.iterations = iter;
+  S->info.relres = error;
   return HOR_OK;
 }
 
@@ -489,17 +572,34 @@ extern "C" int hor_solver_gmres(hor_solver* S, const double* b, double* x, int m
 }
 
 // Flexible GMRES(m) with an optional right preconditioner (a trained
-// point-cloud network); the final residual is always the FP64 true residual.
+// point-cloud network) under the solver's policy; the final residual is always
+// the FP64 true residual.
 extern "C" int hor_solver_fgmres(hor_solver* S, const double* b, double* x, int m, double tol, int maxit,
                                  hor_mpnn* precond, int* iters, double* relres, double* hist, double* log_z_dev,
                                  double* log_az_dev, int log_max, int* n_logged) {
   if (!S || !b || !x || m < 1 || maxit < 0 || log_max < 0 || ((log_z_dev == nullptr) != (log_az_dev == nullptr)))
     return set_error(HOR_E_ARG, "hor_solver_fgmres: bad argument");
   PrecondOp M;
-  M.net = precond;
+  M.net = S->policy.force_plain ? nullptr : precond;
+  M.gate = S->policy.gate != 0;
+  M.ratio = S->policy.gate_ratio;
   if (precond) HOR_TRY(hor_mpnn_check_size(precond, S->n));
   return gmres_core(S, true, M, S->precision == HOR_MIXED, true, b, x, m, tol, maxit, iters, relres, hist,
                     log_z_dev, log_az_dev, log_max, n_logged);
+}
+
+extern "C" int hor_solver_set_precond_policy(hor_solver* S, const hor_precond_policy* p) {
+  if (!S || !p) return set_error(HOR_E_ARG, "hor_solver_set_precond_policy: null");
+  if (!(p->gate_ratio > 0.0) || !std::isfinite(p->gate_ratio))
+    return set_error(HOR_E_ARG, "hor_solver_set_precond_policy: gate_ratio must be finite and > 0");
+  S->policy = *p;
+  return HOR_OK;
+}
+
+extern "C" int hor_solver_fgmres_info(hor_solver* S, hor_fgmres_info* info) {
+  if (!S || !info) return set_error(HOR_E_ARG, "hor_solver_fgmres_info: null");
+  *info = S->info;
+  return HOR_OK;
 }
 
 static int apply_precond(hor_solver* S, const PrecondOp& M, const double* v, double* z) {

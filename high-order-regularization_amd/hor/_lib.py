@@ -33,6 +33,16 @@ class SurfaceDesc(C.Structure):
                 ("nrst", _P), ("x0", _P), ("n0", _P), ("b", _P), ("i123", _P)]
 
 
+class PrecondPolicy(C.Structure):
+    _fields_ = [("force_plain", _I), ("gate", _I), ("gate_ratio", _D)]
+
+
+class FgmresInfo(C.Structure):
+    _fields_ = [("iterations", _I), ("learned_steps", _I), ("plain_steps", _I), ("nonfinite_outputs", _I),
+                ("cycles_gated_on", _I), ("cycles_gated_off", _I), ("monitor_stops", _I), ("gate_misfit_identity", _D),
+                ("gate_misfit_network", _D), ("relres", _D)]
+
+
 # name -> (restype, argtypes)
 _SIGS = {
     "hor_last_error": (C.c_char_p, []),
@@ -74,6 +84,8 @@ _SIGS = {
     "hor_solver_interp_g0": (_I, [_P, _I, _P, _P]),
     "hor_solver_gmres": (_I, [_P, _P, _P, _I, _D, _I, _IP, _DP, _P]),
     "hor_solver_fgmres": (_I, [_P, _P, _P, _I, _D, _I, _P, _IP, _DP, _P, _P, _P, _I, _IP]),
+    "hor_solver_set_precond_policy": (_I, [_P, C.POINTER(PrecondPolicy)]),
+    "hor_solver_fgmres_info": (_I, [_P, C.POINTER(FgmresInfo)]),
     "hor_mpnn_param_count": (_I, []),
     "hor_mpnn_create": (_I, [_P, _I, C.POINTER(_P)]),
     "hor_mpnn_destroy": (_I, [_P]),

@@ -297,25 +297,22 @@ def scale_output(p, alpha):
     return q
 
 
-def solve(S, b, net=None, m=15, tol=1e-8, maxit=200, learned_maxit=None):
-    """GMRES with the learned right preconditioner and plain GMRES as the
-    fallback: flexible GMRES with ``net`` for at most ``learned_maxit``
-    iterations (default 2m); if the FP64 residual is still above tol, plain
-    GMRES solves for the correction of the remaining residual.  Returns x and
-    a dict with the iteration counts of both phases and the final FP64
-    residual |b - A x|/|b|."""
+def solve(S, b, net=None, m=15, tol=1e-8, maxit=200, gate=True, gate_ratio=0.5, force_plain=False):
+    """Flexible GMRES with the learned right preconditioner, plain GMRES as
+    the fallback.  The solver's policy (hor_solver_set_precond_policy) makes
+    the first Arnoldi step of every restart cycle plain and uses the network
+    for the rest of the cycle only if, on that step's pair (v0, A v0), it
+    inverts A better than the identity does (gate); a non-finite network
+    output is replaced by the plain step and ends the network's use in the
+    solve; force_plain ignores the network.  Returns x and a dict with the
+    iteration count, the learned and plain steps, the gate decisions and the
+    final FP64 residual |b - A x|/|b|."""
     b = f64(b, S.n)
-    if net is None:
-        x, it, _, res, _ = S.fgmres(b, m=m, tol=tol, maxit=maxit)
-        return x, dict(learned_iters=0, plain_iters=it, relres=res, fallback=False)
-    lm = 2 * m if learned_maxit is None else learned_maxit
-    x, it1, _, res1, _ = S.fgmres(b, m=m, tol=tol, maxit=min(lm, maxit), precond=net)
-    if res1 <= tol or it1 >= maxit:
-        return x, dict(learned_iters=it1, plain_iters=0, relres=res1, fallback=False)
-    r = b - S.matvec(x)                      # FP64 residual
-    bn = float(np.linalg.norm(b))
-    rn = float(np.linalg.norm(r))
-    d, it2, _, res2, _ = S.fgmres(r, m=m, tol=tol * bn / rn, maxit=maxit - it1)
-    x = x + d
-    res = float(np.linalg.norm(b - S.matvec(x)) / bn)
-    return x, dict(learned_iters=it1, plain_iters=it2, relres=res, fallback=True)
+    S.set_precond_policy(force_plain=force_plain, gate=gate, gate_ratio=gate_ratio)
+    try:
+        x, it, _, res, _ = S.fgmres(b, m=m, tol=tol, maxit=maxit, precond=net)
+        info = S.last_info()
+    finally:
+        S.set_precond_policy()
+    info.update(iterations=it, relres=res, fallback=bool(net is not None and info["learned_steps"] == 0))
+    return x, info

@@ -106,6 +106,19 @@ class Solver:
                                            ptr(hist), lz, laz, int(lmax), C.byref(nl)), "hor_solver_fgmres")
         return x, it.value, hist[: it.value], res.value, nl.value
 
+    def set_precond_policy(self, force_plain=False, gate=True, gate_ratio=0.5):
+        """How fgmres uses a network (hor_solver_set_precond_policy):
+        force_plain ignores it; gate decides per restart cycle from the
+        cycle's first (plain) Arnoldi pair whether the network is used."""
+        p = _lib.PrecondPolicy(int(bool(force_plain)), int(bool(gate)), float(gate_ratio))
+        check(_lib.lib().hor_solver_set_precond_policy(self.h, C.byref(p)), "hor_solver_set_precond_policy")
+
+    def last_info(self):
+        """Record of the last gmres/fgmres call (hor_solver_fgmres_info)."""
+        info = _lib.FgmresInfo()
+        check(_lib.lib().hor_solver_fgmres_info(self.h, C.byref(info)), "hor_solver_fgmres_info")
+        return {k: getattr(info, k) for k, _ in _lib.FgmresInfo._fields_}
+
     def gmres(self, b, m=15, tol=1e-10, maxit=100):
         b = f64(b, self.n)
         x = np.zeros(self.n)
@@ -114,3 +127,36 @@ class Solver:
         check(_lib.lib().hor_solver_gmres(self.h, ptr(b), ptr(x), int(m), float(tol), int(maxit), C.byref(it),
                                           C.byref(res), ptr(hist)), "hor_solver_gmres")
         return x, it.value, hist[: it.value], res.value
+
+
+SEED = 20261203
+
+
+def two_sphere_problem(h, gap, seed=SEED, ctx=None, radius=0.5):
+    """The two-sphere workload of the second-kind solve: spheres of the given
+    radius with centres (0, 0, -+(2 radius + gap)/2), the solver's surface
+    dicts (nearest data of each surface's points relative to the other
+    surface, the surface force f and its value f0 at each nearest point,
+    surface.cpp:450-559) and a seeded rigid-body right-hand side
+    b = U_k + W_k x (x - c_k) with U, W ~ U(-1, 1) per sphere (numpy PCG64).
+    The reference's own right-hand side is b = -2/3 SL[f] (main.cpp:199-207),
+    Solver.eval_sl(f1, f2, f01, f02).  Quadrature and nearest data are built
+    on the GPU of ctx when given, else on the host (identical outputs).
+    Returns DEL, the two bodies, the two surface dicts and the rigid b."""
+    from . import geometry as G
+    DEL = G.delta(h)
+    zc = radius + gap / 2
+    bodies = [G.Body((radius,) * 3, (0.0, 0.0, -zc)), G.Body((radius,) * 3, (0.0, 0.0, zc))]
+    surfs = [b.surface(h, ctx=ctx) for b in bodies]
+    pd = []
+    for k in range(2):
+        t = bodies[1 - k].nearest(h, DEL, surfs[1 - k], surfs[k]["x"], with_force=True, ctx=ctx)
+        pd.append(dict(surfs[k], ctr=bodies[k].ctr, nrst=t["nrst"], x0=t["x0"], n0=t["n0"], b=t["b"],
+                       i123=t["i123"], f=bodies[k].force(surfs[k]["x"], surfs[k]["nrml"]), f0=t["f0"]))
+    rng = np.random.default_rng(seed)
+    rigid = []
+    for k in range(2):
+        U, Wv = rng.uniform(-1, 1, 3), rng.uniform(-1, 1, 3)
+        x = pd[k]["x"].reshape(-1, 3)
+        rigid.append((U + np.cross(Wv, x - bodies[k].ctr)).ravel())
+    return DEL, bodies, pd, np.concatenate(rigid)

@@ -261,7 +261,8 @@ int hor_solver_interp_g0(hor_solver* solver, int k, const double* u, double* g0)
  * reference.  Mixed-precision solver: the Arnoldi steps use the mixed operator,
  * every restart residual and the final check use the FP64 operator, a cycle
  * whose FP64 residual is above tol continues, and *relres is that FP64
- * residual. */
+ * residual, also when the solve stops at maxit.  A non-finite value in b, in a
+ * residual or in an Arnoldi step ends the solve with HOR_E_NAN. */
 int hor_solver_gmres(hor_solver* solver, const double* b, double* x, int m,
                      double tol, int maxit, int* iters, double* relres,
                      double* hist);
@@ -315,6 +316,43 @@ int hor_solver_fgmres(hor_solver* solver, const double* b, double* x, int m,
                       double tol, int maxit, hor_mpnn* precond, int* iters,
                       double* relres, double* hist, double* log_z_dev,
                       double* log_az_dev, int log_max, int* n_logged);
+
+/* How hor_solver_fgmres uses its network (no counterpart in the reference).
+ *   force_plain: nonzero ignores the network: plain GMRES (the host-forced
+ *     fallback).
+ *   gate: nonzero (the default) makes the first Arnoldi step of every restart
+ *     cycle plain and lets its pair (v0, w = A v0) decide the rest of the
+ *     cycle: with the scale-free misfit e(X)^2 = 1 - (X(w).v0)^2/(|X(w)|^2
+ *     |v0|^2) of an approximate inverse X, the network is used for the
+ *     cycle's remaining steps only if e(M) < gate_ratio * e(I).  A cycle whose
+ *     gate rejects the network is exactly plain GMRES.  While the network is
+ *     in use, a step that reduces the residual estimate by less than the
+ *     plain first step did (by the factor e(I)) ends its use for the rest of
+ *     the cycle.  With gate = 0 every step uses the network.
+ * Every network output is screened: a non-finite one is replaced by the plain
+ * step and the network is not used again in that solve.  Defaults
+ * {0, 1, 0.5}. */
+typedef struct {
+  int force_plain;
+  int gate;
+  double gate_ratio;
+} hor_precond_policy;
+int hor_solver_set_precond_policy(hor_solver* solver, const hor_precond_policy* policy);
+
+/* Record of the last hor_solver_gmres / hor_solver_fgmres on this solver. */
+typedef struct {
+  int iterations;
+  int learned_steps;        /* Arnoldi steps with z = M(v) */
+  int plain_steps;          /* Arnoldi steps with z = v (fgmres) */
+  int nonfinite_outputs;    /* network outputs rejected by the finite screen */
+  int cycles_gated_on;      /* restart cycles whose gate accepted the network */
+  int cycles_gated_off;     /* ... rejected it */
+  int monitor_stops;        /* cycles where the progress monitor ended its use */
+  double gate_misfit_identity;  /* e(I) and e(M) of the last gate test (-1: none) */
+  double gate_misfit_network;
+  double relres;            /* the returned relative residual */
+} hor_fgmres_info;
+int hor_solver_fgmres_info(hor_solver* solver, hor_fgmres_info* info);
 
 /* ------------------------------------------------------------------------
  * Device memory helpers (for callers who keep data resident).

@@ -206,10 +206,12 @@ class Oracle:
             tp.ctr[k] = float(ctr[k])
         return tp
 
-    def sl_on_tree(self, tp, DEL, src):
-        out = np.zeros(3 * src["n"])
+    def sl_on_tree(self, tp, DEL, src, nt=None):
+        """On-surface treecode SL at the first nt source points (all by default)."""
+        nt = src["n"] if nt is None else int(nt)
+        out = np.zeros(3 * nt)
         self.lib.orc_sl_on_tree(C.byref(tp), _D(DEL), src["n"], _p(src["x"]), _p(src["nrml"]),
-                                _p(src["area"]), _p(src["f"]), src["n"], _p(src["x"]), _p(out))
+                                _p(src["area"]), _p(src["f"]), nt, _p(src["x"]), _p(out))
         return out
 
     def sl_off_tree(self, tp, DEL, src, tgt):
@@ -219,10 +221,12 @@ class Oracle:
                                  _p(tgt["b"]), _p(tgt["n0"]), _p(tgt["f0"]), _p(out))
         return out
 
-    def dl_on_tree(self, tp, DEL, src):
-        out = np.zeros(3 * src["n"])
+    def dl_on_tree(self, tp, DEL, src, nt=None):
+        """On-surface treecode DL at the first nt source points (all by default)."""
+        nt = src["n"] if nt is None else int(nt)
+        out = np.zeros(3 * nt)
         self.lib.orc_dl_on_tree(C.byref(tp), _D(DEL), src["n"], _p(src["x"]), _p(src["nrml"]),
-                                _p(src["area"]), _p(src["g"]), src["n"], _p(src["x"]), _p(out))
+                                _p(src["area"]), _p(src["g"]), nt, _p(src["x"]), _p(out))
         return out
 
     def dl_off_tree(self, tp, DEL, src, tgt):

@@ -134,3 +134,49 @@ def test_training_reduces_loss_and_preconditions(ctx):
         s.free()
     net.close()
     S.close()
+
+
+def test_nonfinite_network_output_falls_back(ctx):
+    """Parameters that overflow FP32 in the hidden layers make the network
+    output non-finite; the solver screens every output, takes the plain step
+    instead, stops using the network, and ends on the FP64 residual with the
+    plain GMRES iteration count."""
+    h, DEL, bodies, pd, _ = _setup(N=8)
+    S = Solver(ctx, h, DEL, pd[0], pd[1])
+    b = np.random.default_rng(9).uniform(-1, 1, S.n)
+    x0, st0 = M.solve(S, b, net=None, tol=1e-10, maxit=100)
+    net = M.Network(ctx, k=16, params=np.full(M.param_count(), 1e30, np.float32))
+    net.set_geometry(pd, bodies)
+    z = net.forward(b)
+    assert not np.all(np.isfinite(z))          # the injected failure
+    for gate in (False, True):
+        x1, st1 = M.solve(S, b, net=net, tol=1e-10, maxit=100, gate=gate)
+        print(f"gate={gate}: plain {st0['iterations']} iterations; with the overflowing network "
+              f"{st1['iterations']} ({st1['nonfinite_outputs']} non-finite outputs, {st1['learned_steps']} learned "
+              f"steps), FP64 residual {st1['relres']:.2e}")
+        assert st1["nonfinite_outputs"] == 1 and st1["learned_steps"] == 0
+        assert st1["iterations"] == st0["iterations"]
+        assert st1["relres"] <= 1e-10
+        assert np.linalg.norm(b - S.matvec(x1)) / np.linalg.norm(b) <= 1.01e-10
+        assert np.linalg.norm(x1 - x0) <= 1e-12 * np.linalg.norm(x0)
+    net.close()
+    S.close()
+
+
+def test_forced_plain_policy(ctx):
+    """force_plain ignores the network: the solve is plain GMRES exactly."""
+    h, DEL, bodies, pd, p = _setup(N=8)
+    S = Solver(ctx, h, DEL, pd[0], pd[1])
+    b = np.random.default_rng(10).uniform(-1, 1, S.n)
+    x0, st0 = M.solve(S, b, net=None, tol=1e-10, maxit=100)
+    net = M.Network(ctx, k=16, params=p)
+    net.set_geometry(pd, bodies)
+    x1, st1 = M.solve(S, b, net=net, tol=1e-10, maxit=100, force_plain=True)
+    assert st1["learned_steps"] == 0 and st1["iterations"] == st0["iterations"]
+    assert np.array_equal(x1, x0)
+    x2, st2 = M.solve(S, b, net=net, tol=1e-10, maxit=100, gate=False)
+    print(f"plain {st0['iterations']}, forced plain {st1['iterations']}, ungated network {st2['iterations']} "
+          f"({st2['learned_steps']} learned steps)")
+    assert st2["learned_steps"] == st2["iterations"] and st2["relres"] <= 1e-10
+    net.close()
+    S.close()

@@ -111,3 +111,45 @@ def test_mixed_gmres_ends_on_fp64_residual(ctx):
         assert np.linalg.norm(x - x64) <= 10 * tol * np.linalg.norm(x64) / 0.3
     s64.close()
     smx.close()
+
+
+def test_gmres_nonfinite_rhs_is_an_error(ctx):
+    """A NaN or Inf in b ends the solve with HOR_E_NAN (no iteration on
+    non-finite data)."""
+    h, DEL, bodies, surfs, tgts = sphere_pair(N=8, gap=0.1)
+    pd = problem_dicts(bodies, surfs, tgts)
+    sol = S.Solver(ctx, h, DEL, pd[0], pd[1])
+    for bad in (np.nan, np.inf):
+        b = np.random.default_rng(6).uniform(-1, 1, sol.n)
+        b[17] = bad
+        for call in (lambda: sol.gmres(b, m=15, tol=1e-8, maxit=50),
+                     lambda: sol.fgmres(b, m=15, tol=1e-8, maxit=50)):
+            with pytest.raises(hor.HorError) as e:
+                call()
+            assert e.value.code == -4  # HOR_E_NAN
+    # the solver stays usable
+    b = np.random.default_rng(6).uniform(-1, 1, sol.n)
+    _, it, _, res = sol.gmres(b, m=15, tol=1e-8, maxit=50)
+    assert res <= 1e-8
+    sol.close()
+
+
+def test_maxit_returns_the_true_residual(ctx):
+    """A mixed-precision GMRES or a flexible GMRES stopped by maxit reports
+    the FP64 true residual |b - A x|/|b|, not the Arnoldi estimate."""
+    h, DEL, bodies, surfs, tgts = sphere_pair(32, gap=0.1)
+    pd = problem_dicts(bodies, surfs, tgts)
+    tp = hor.TreeParams(6, 500, 0.6)
+    s64 = S.Solver(ctx, h, DEL, pd[0], pd[1], method=hor.TREE, precision=hor.FP64, tree1=tp, tree2=tp)
+    smx = S.Solver(ctx, h, DEL, pd[0], pd[1], method=hor.TREE, precision=hor.MIXED, tree1=tp, tree2=tp)
+    b = np.random.default_rng(8).uniform(-1, 1, s64.n)
+    for name, call in (("mixed gmres", lambda mi: smx.gmres(b, m=15, tol=1e-14, maxit=mi)[::3]),
+                       ("fgmres", lambda mi: s64.fgmres(b, m=15, tol=1e-14, maxit=mi)[::3]),
+                       ("mixed fgmres m=2", lambda mi: smx.fgmres(b, m=2, tol=1e-14, maxit=mi)[::3])):
+        for maxit in (3, 4):
+            x, res = call(maxit)
+            true = np.linalg.norm(b - s64.matvec(x)) / np.linalg.norm(b)
+            print(f"{name}, maxit {maxit}: reported {res:.6e}, true {true:.6e}")
+            assert abs(res - true) <= 1e-9 * true
+    s64.close()
+    smx.close()
