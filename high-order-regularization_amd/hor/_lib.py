@@ -61,6 +61,7 @@ _SIGS = {
     "hor_sl_onsurf": (_I, [_P, _D, _I, _P, _P, _P, _P, _I, _P, _P]),
     "hor_dl_offsurf": (_I, [_P, _D, _I, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P]),
     "hor_dl_onsurf": (_I, [_P, _D, _I, _P, _P, _P, _P, _I, _P, _P]),
+    "hor_dl_onsurf_dev_noalloc": (_I, [_P, _D, _I, _P, _P, _P, _P, _I, _P, _P]),
     "hor_sl_onsurf_tree": (_I, [_P, _P, C.POINTER(TreeParams), _D, _I, _P, _P, _P, _P, _I, _P, _P]),
     "hor_sl_offsurf_tree": (_I, [_P, _P, C.POINTER(TreeParams), _D, _I, _P, _P, _P, _P, _I, _P,
                                  _P, _P, _P, _P]),

@@ -190,6 +190,24 @@ int hor_dl_offsurf_tree(hor_ctx* ctx, const double* ctr, const hor_tree_params* 
                         const double* tx, const double* tb, const double* tx0,
                         const double* tn0, const double* tg0, double* out);
 
+/* eval_Stokes_DL_onSurf (stresslets.h:21-24) as hor_dl_onsurf computes it,
+ * with every pointer a device pointer (sx/sn/sg 3*ns, sa ns, tx 3*nt, out 3*nt;
+ * target i is source i) and NO device memory allocated: no counterpart in the
+ * reference; it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Its
+ * intermediate buffers live in one 32 MiB array of device memory that each GPU
+ * holds from the library's first use until the process ends; sizes whose
+ * buffers do not fit (above about ns = nt = 62000) are rejected with HOR_E_ARG.
+ * Calls on one GPU share that array: calls that overlap on one GPU (from two
+ * contexts or threads) race and must be serialized by the caller.  Sources and
+ * targets keep the caller's order (no spatial sort), so the result equals
+ * hor_dl_onsurf's up to the order of the sums.  Synchronous, like the rest of
+ * the library. */
+int hor_dl_onsurf_dev_noalloc(hor_ctx* ctx, double DEL, int ns, const double* sx_dev,
+                              const double* sn_dev, const double* sa_dev,
+                              const double* sg_dev, int nt, const double* tx_dev,
+                              double* out_dev);
+
 /* stokes_pressure_7ord (extend_to_grid/stokes.h:24-28) at nt targets: sf the
  * surface force, tb/tn0/tf0 the signed distance, nearest normal and the
  * force there; out: nt pressures. */
