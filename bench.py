@@ -45,7 +45,9 @@ VALU_ISSUE = 256 * 4 * 16 * 2.4e9   # VALU lane-instructions/s: 256 CUs x 4 SIMD
 # its reduced form dx.w (w = -a f + (f0.n0) a n + (n0 x f0) x a n), fewer
 # operations than the reference's normal + tangential expression: plain 36,
 # regularized 63.
-OPS = {"sl_plain": 30, "sl_reg": 61, "dl_plain": 30, "dl_reg": 127, "p_plain": 36, "p_reg": 63}
+OPS = {"sl_plain": 30, "sl_reg": 61, "dl_plain": 30, "dl_reg": 127, "p_plain": 36, "p_reg": 63,
+       "dl_on_reg": 52, "dl_proxy": 51, "dl_moment": 37}
+PEAK_FP32 = 157.3e12         # MI355X FP32 (vector and matrix) FLOP/s
 
 
 def log(*a):
@@ -277,6 +279,19 @@ def gmres_workload(h=1.0 / 128.0, gap=0.1, P=8, N0=4000, theta=0.6, tol=1e-8, se
         for _ in range(3):
             S.matvec(b_rigid)
         out.setdefault("matvec_ms", {})[name] = 1e3 * (time.perf_counter() - t0) / 3
+    if abs(h - 1.0 / 256.0) < 1e-12:
+        ud, yd = ctx.alloc(b_rigid.nbytes), ctx.alloc(b_rigid.nbytes)
+        ctx.upload(ud, b_rigid)
+        freg = onsurf_reg_fraction(S64, pd, DEL)
+        counted = profile_tree()
+        out["roofline"] = {name: tree_roofline(S, phase_mean(S, ud, yd, 3), freg, name == "mixed", counted)
+                           for name, S in solvers.items()}
+        out["roofline"]["note"] = ("phase times from HIP events on the solver's stream around each plan's "
+                                   "k_moments, k_leaf and k_far launches, summed over the four DL plans of a "
+                                   "matvec and averaged over 3 matvecs; counted figures from "
+                                   "profiles/tree_matvec_h256_pmc.json")
+        ctx.free(ud)
+        ctx.free(yd)
     st = S64.stats()
     mv_s = out["matvec_ms"]["fp64"] * 1e-3
     out["matvec_work"] = dict(st, direct_equivalent_pairs_per_s=st["direct_pairs"] / mv_s,
@@ -354,6 +369,81 @@ def gmres_workload(h=1.0 / 128.0, gap=0.1, P=8, N0=4000, theta=0.6, tol=1e-8, se
     return out
 
 
+def onsurf_reg_fraction(S, pd, DEL, sample=2000, seed=3):
+    """Fraction of the on-surface near-leaf pairs that are regularized
+    (r/delta <= 8), from the mean number of same-surface sources within
+    8 delta of a seeded sample of targets (k-d tree)."""
+    from scipy.spatial import cKDTree
+    rng = np.random.default_rng(seed)
+    nreg = 0.0
+    for k in range(2):
+        x = pd[k]["x"].reshape(-1, 3)
+        idx = rng.choice(x.shape[0], min(sample, x.shape[0]), replace=False)
+        cnt = cKDTree(x).query_ball_point(x[idx], 8.0 * DEL, return_length=True)
+        nreg += float(np.mean(cnt)) * x.shape[0]
+    near_on = S.plan_stats(0)["near_pairs"] + S.plan_stats(1)["near_pairs"]
+    return min(1.0, nreg / max(near_on, 1.0))
+
+
+def tree_roofline(S, phase, freg, mixed, counted=None):
+    """Roofline figures of one treecode matvec's kernels from its phase times
+    (HIP events, ms) and the plans' work: the DL far field k_far (51 flops per
+    proxy interaction), the on-surface near sums k_leaf<DL_ON> (52 flops per
+    regularized pair, 30 per plain pair; the fraction regularized sampled) and
+    the moments (37 flops per moment term, FP64 matrix cores)."""
+    st = [S.plan_stats(k) for k in range(4)]
+    proxy = sum(x["proxy_pairs"] for x in st)
+    near = sum(x["near_pairs"] for x in st)
+    mom = sum(x["moment_terms"] for x in st)
+    peak_far = PEAK_FP32 if mixed else PEAK_FP64
+    far_flops = proxy * OPS["dl_proxy"]
+    near_flops = near * (freg * OPS["dl_on_reg"] + (1 - freg) * OPS["dl_plain"])
+    mom_flops = mom * OPS["dl_moment"]
+    out = {
+        "far": {"kernel": "k_far<true, %s>" % ("true" if mixed else "false"), "bound": "valu",
+                "compute": "FP32 vector ALU, FP64 row sums" if mixed else "FP64 vector ALU",
+                "ms": phase["far_ms"], "flop": far_flops, "achieved": far_flops / (phase["far_ms"] * 1e-3) / 1e12,
+                "peak": peak_far / 1e12, "unit": "TFLOP/s",
+                "frac": far_flops / (phase["far_ms"] * 1e-3) / peak_far},
+        "near": {"kernel": "k_leaf (DL_ON and DL_OFF)", "bound": "valu", "ms": phase["near_ms"],
+                 "flop": near_flops, "regularized_fraction": freg,
+                 "achieved": near_flops / (phase["near_ms"] * 1e-3) / 1e12, "peak": PEAK_FP64 / 1e12,
+                 "unit": "TFLOP/s", "frac": near_flops / (phase["near_ms"] * 1e-3) / PEAK_FP64},
+        "moments": {"kernel": "k_moments (FP64 matrix cores) + reduction", "bound": "mfma", "ms": phase["moments_ms"],
+                    "flop": mom_flops, "achieved": mom_flops / max(phase["moments_ms"] * 1e-3, 1e-12) / 1e12,
+                    "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
+                    "frac": mom_flops / max(phase["moments_ms"] * 1e-3, 1e-12) / PEAK_FP64},
+        "work": {"proxy_interactions": proxy, "near_pairs": near, "moment_terms": mom},
+    }
+    if counted:
+        for key, name in (("far", out["far"]["kernel"]), ("near", "k_leaf<2, %s>" % ("true" if mixed else "false"))):
+            c = counted.get(name)
+            if c:
+                out[key]["counted"] = {k: c[k] for k in ("fp64_tflops", "fp64_frac_of_peak", "fp32_tflops",
+                                                         "valu_issue_frac", "hbm_bytes_per_launch") if k in c}
+    return out
+
+
+def profile_tree():
+    """Per-kernel counters of the h=1/256 treecode matvec from the committed
+    rocprofv3 summary (VALU FLOPs, issue rate, HBM bytes)."""
+    path = os.path.join(ROOT, "profiles", "tree_matvec_h256_pmc.json")
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f).get("kernels")
+    return None
+
+
+def phase_mean(S, u_dev, y_dev, reps):
+    """Mean phase times (ms) over reps device matvecs."""
+    acc = {"moments_ms": 0.0, "near_ms": 0.0, "far_ms": 0.0}
+    for _ in range(reps):
+        S.matvec_dev(u_dev, y_dev)
+        for k, v in S.phase_timing().items():
+            acc[k] += v / reps
+    return acc
+
+
 def matvec_workload(h=1.0 / 256.0, gap=0.1, P=10, N0=8000, theta=0.6, reps=3, seed=SEED):
     """Only the two-sphere treecode matvecs (the GMRES operator): reps FP64
     and reps mixed-precision matvecs of the seeded rigid-body vector, after one
@@ -375,6 +465,8 @@ def matvec_workload(h=1.0 / 256.0, gap=0.1, P=10, N0=8000, theta=0.6, reps=3, se
             S.matvec_dev(ud, yd)
         out[f"{name}_matvec_ms"] = 1e3 * (time.perf_counter() - t0) / reps
         out[f"{name}_work"] = S.stats()
+        out[f"{name}_phase_ms"] = S.phase_timing()
+        out[f"{name}_plan_stats"] = [S.plan_stats(k) for k in range(4)]
         log(f"{name}: {out[f'{name}_matvec_ms']:.1f} ms per matvec")
         ctx.free(ud)
         ctx.free(yd)
@@ -605,6 +697,7 @@ def main():
                                "|b|<=0.3, FP64 direct SL_OFF + DL_OFF (one fused pass)",
                    "sources": ns, "targets": nt, "delta": DEL, "pairs_per_step": pairs_per_step},
         "roofline": roof,
+        "gmres_roofline": (gm or {}).get("h256", {}).get("roofline"),
         "cpu_baseline": cpu,
         "separate_ops": {"ms_per_step": sep_ms, "value": pairs_per_step / (sep_ms * 1e-3),
                          "max_rel_diff_vs_fused": fused_vs_separate,

@@ -71,6 +71,13 @@ struct TreePlan {
   // reduction
   DBuf<int> slot_start;      // sorted target order, nt+1
   DBuf<double> slots;        // 3*nslot
+  // phase events of the last evaluation: before the moments, after the
+  // moments, after the near sums, after the far field
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  ~TreePlan() {
+    for (hipEvent_t& e : ev)
+      if (e) (void)hipEventDestroy(e);
+  }
 };
 
 // ---------------------------------------------------------------------------
@@ -559,6 +566,7 @@ int tree_plan_create(hor_plan* P, const double* sx, const double* sn, const doub
   hipStream_t s = P->ctx->stream;
   P->tree.reset(new TreePlan());
   TreePlan& TP = *P->tree;
+  for (hipEvent_t& e : TP.ev) HOR_HIP_TRY(hipEventCreate(&e));
   Tree& T = TP.T;
   tree_build(T, P->ns, sx, ctr, tp.P, tp.N0, tp.theta);
   TP.np1 = tp.P + 1;
@@ -776,6 +784,7 @@ int tree_plan_eval(hor_plan* P, const double* dens_dev, const double* d0_dev, do
   else
     HOR_HIP_TRY(hipMemcpyAsync(P->tsub2.p, P->tsub.p, sizeof(double) * 3 * P->ntp, hipMemcpyDeviceToDevice, s));
   HOR_HIP_TRY(hipEventRecord(c->ev[2], s));
+  HOR_HIP_TRY(hipEventRecord(TP.ev[0], s));
   // 1. moments (unless another plan computes them)
   if (TP.nfar_nodes > 0 && !TP.mom_owner) {
     hipLaunchKernelGGL(k_moment_rhs, dim3((P->ns + kBlock - 1) / kBlock), dim3(kBlock), 0, s, P->ns, P->nsp,
@@ -804,6 +813,7 @@ int tree_plan_eval(hor_plan* P, const double* dens_dev, const double* d0_dev, do
                        mixed ? TP.mom32.p : nullptr, dl ? kNc32Dl : kNc32Sl);
     HOR_HIP_TRY(hipGetLastError());
   }
+  HOR_HIP_TRY(hipEventRecord(TP.ev[1], s));
   // 2. near leaves
   if (TP.n_near_items > 0) {
     LeafArgs A;
@@ -821,6 +831,7 @@ int tree_plan_eval(hor_plan* P, const double* dens_dev, const double* d0_dev, do
     }
     HOR_HIP_TRY(hipGetLastError());
   }
+  HOR_HIP_TRY(hipEventRecord(TP.ev[2], s));
   // 3. far field
   if (TP.n_far_items > 0) {
     FarArgs F;
@@ -841,12 +852,27 @@ int tree_plan_eval(hor_plan* P, const double* dens_dev, const double* d0_dev, do
     HOR_HIP_TRY(hipGetLastError());
   }
   HOR_HIP_TRY(hipEventRecord(c->ev[3], s));
+  HOR_HIP_TRY(hipEventRecord(TP.ev[3], s));
   // 4. per-target reduction
   hipLaunchKernelGGL(k_tree_finalize, dim3((P->nt + kBlock - 1) / kBlock, 3), dim3(kBlock), 0, s, P->nt, P->ntp,
                      TP.nslot, P->tperm.p, TP.slot_start.p, TP.slots.p,
                      dl ? -6.0 / (8.0 * kPi) : 1.0 / (8.0 * kPi),   // KITC.cpp:1395, 1612
                      dl ? P->xi.p : nullptr, dl ? P->tsub2.p : nullptr, out_dev, base, coef, accumulate);
   HOR_HIP_TRY(hipGetLastError());
+  return HOR_OK;
+}
+
+// Kernel time of each phase of the plan's last evaluation (HIP events on the
+// context's stream; call after the stream has synchronized): moments (moment
+// right-hand side, GEMM and reduction), near sums (k_leaf), far field (k_far).
+int tree_plan_phase_ms(hor_plan* P, double ms[3]) {
+  if (!P->tree) return set_error(HOR_E_STATE, "not a treecode plan");
+  const TreePlan& TP = *P->tree;
+  for (int k = 0; k < 3; k++) {
+    float t = 0;
+    HOR_HIP_TRY(hipEventElapsedTime(&t, TP.ev[k], TP.ev[k + 1]));
+    ms[k] = t;
+  }
   return HOR_OK;
 }
 

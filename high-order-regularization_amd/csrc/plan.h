@@ -66,5 +66,8 @@ int tree_plan_create(hor_plan* P, const double* sx, const double* sn, const doub
 int tree_plan_share_moments(hor_plan* owner, hor_plan* user, const double* sx);
 int tree_plan_eval(hor_plan* P, const double* dens_dev, const double* d0_dev, double* out_dev,
                    const double* base, double coef, int accumulate);
+// Kernel time of each phase of a treecode plan's last evaluation: moments,
+// near sums, far field (ms).
+int tree_plan_phase_ms(hor_plan* P, double ms[3]);
 
 }  // namespace hor

@@ -273,6 +273,35 @@ extern "C" int hor_solver_stats(hor_solver* S, double* direct_pairs, double* nea
   return HOR_OK;
 }
 
+// Work of one of the four DL plans: 0, 1 the on-surface plans of surfaces 1
+// and 2, 2, 3 the plans of surface 1 and 2 sources at the other surface.
+extern "C" int hor_solver_plan_stats(hor_solver* S, int which, double* direct_pairs, double* near_pairs,
+                                     double* proxy_pairs, double* moment_terms) {
+  if (!S || which < 0 || which > 3) return set_error(HOR_E_ARG, "hor_solver_plan_stats: bad argument");
+  hor_plan* plans[4] = {S->on[0], S->on[1], S->off[0], S->off[1]};
+  return hor_plan_stats(plans[which], direct_pairs, near_pairs, proxy_pairs, moment_terms);
+}
+
+// Kernel time by phase of the last evaluation of the solver's four DL plans
+// (the operator of its precision), summed: moments, near sums, far field.
+extern "C" int hor_solver_phase_timing(hor_solver* S, double* moments_ms, double* near_ms, double* far_ms) {
+  if (!S) return set_error(HOR_E_ARG, "hor_solver_phase_timing: null");
+  if (S->method != HOR_TREE) return set_error(HOR_E_STATE, "hor_solver_phase_timing: treecode solvers only");
+  HOR_HIP_TRY(hipSetDevice(S->ctx->device));
+  HOR_HIP_TRY(hipStreamSynchronize(S->ctx->stream));
+  double acc[3] = {0, 0, 0};
+  hor_plan* plans[4] = {S->on[0], S->on[1], S->off[0], S->off[1]};
+  for (hor_plan* P : plans) {
+    double ms[3];
+    HOR_TRY(tree_plan_phase_ms(P, ms));
+    for (int k = 0; k < 3; k++) acc[k] += ms[k];
+  }
+  if (moments_ms) *moments_ms = acc[0];
+  if (near_ms) *near_ms = acc[1];
+  if (far_ms) *far_ms = acc[2];
+  return HOR_OK;
+}
+
 extern "C" int hor_solver_size(hor_solver* S, int* n) {
   if (!S || !n) return set_error(HOR_E_ARG, "null");
   *n = S->n;

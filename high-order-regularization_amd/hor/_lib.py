@@ -79,6 +79,8 @@ _SIGS = {
     "hor_solver_destroy": (_I, [_P]),
     "hor_solver_size": (_I, [_P, _IP]),
     "hor_solver_stats": (_I, [_P, _DP, _DP, _DP, _DP]),
+    "hor_solver_phase_timing": (_I, [_P, _DP, _DP, _DP]),
+    "hor_solver_plan_stats": (_I, [_P, _I, _DP, _DP, _DP, _DP]),
     "hor_solver_matvec": (_I, [_P, _P, _P]),
     "hor_solver_matvec_dev": (_I, [_P, _P, _P]),
     "hor_solver_eval_sl": (_I, [_P, _P, _P, _P, _P, _P]),

@@ -66,6 +66,22 @@ class Solver:
         return dict(direct_pairs=v[0].value, near_pairs=v[1].value, proxy_pairs=v[2].value,
                     moment_terms=v[3].value)
 
+    def plan_stats(self, which):
+        """Work of one DL plan: 0/1 on-surface (DL11, DL22), 2/3 cross-surface
+        (DL12, DL21) (hor_solver_plan_stats)."""
+        v = [C.c_double() for _ in range(4)]
+        check(_lib.lib().hor_solver_plan_stats(self.h, int(which), *[C.byref(x) for x in v]),
+              "hor_solver_plan_stats")
+        return dict(direct_pairs=v[0].value, near_pairs=v[1].value, proxy_pairs=v[2].value,
+                    moment_terms=v[3].value)
+
+    def phase_timing(self):
+        """Kernel ms of the last treecode matvec by phase: moments, near sums,
+        far field (hor_solver_phase_timing)."""
+        v = [C.c_double() for _ in range(3)]
+        check(_lib.lib().hor_solver_phase_timing(self.h, *[C.byref(x) for x in v]), "hor_solver_phase_timing")
+        return dict(moments_ms=v[0].value, near_ms=v[1].value, far_ms=v[2].value)
+
     def matvec(self, u):
         u = f64(u, self.n)
         out = np.zeros(self.n)

@@ -256,6 +256,19 @@ int hor_solver_size(hor_solver* solver, int* n);
 int hor_solver_stats(hor_solver* solver, double* direct_pairs, double* near_pairs,
                      double* proxy_pairs, double* moment_terms);
 
+/* hor_solver_stats of one of the four DL plans: which = 0, 1 the on-surface
+ * plans of surfaces 1 and 2 (DL11, DL22), 2, 3 the sources of surface 1 and 2
+ * at the other surface's points (DL12, DL21). */
+int hor_solver_plan_stats(hor_solver* solver, int which, double* direct_pairs,
+                          double* near_pairs, double* proxy_pairs, double* moment_terms);
+
+/* Kernel time (HIP events on the context's stream) of the phases of the last
+ * matvec of a treecode solver's own operator, summed over its four DL plans:
+ * the moments (k_moment_rhs, k_moments, k_moments_reduce), the near sums
+ * (k_leaf) and the far field (k_far), in ms.  HOR_E_STATE for direct solvers. */
+int hor_solver_phase_timing(hor_solver* solver, double* moments_ms, double* near_ms,
+                            double* far_ms);
+
 /* solve_IE_1step (main.cpp:541-677): Au = u - 2/3 [DL11+DL21; DL12+DL22](u),
  * with g0 interpolated on the 6x6 Monge stencil (main.cpp:916-1046).  Host
  * pointers; the _dev variant takes device pointers of length n. */
