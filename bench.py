@@ -30,6 +30,10 @@ sys.path[:0] = [os.path.join(ROOT, "high-order-regularization_amd"), os.path.joi
 # share); host_cpu() records what the threads ran on.
 os.environ.setdefault("OMP_PROC_BIND", "close")
 os.environ.setdefault("OMP_PLACES", "cores")
+try:   # the process's CPUs, read before OpenMP binds this thread to one of them
+    CPUS_AVAILABLE = len(os.sched_getaffinity(0))
+except AttributeError:
+    CPUS_AVAILABLE = os.cpu_count() or 1
 
 from hor import solver as TS  # noqa: E402
 
@@ -69,10 +73,7 @@ def host_cpu():
                 info[k] = v
     except (OSError, ValueError):
         pass
-    try:
-        avail = len(os.sched_getaffinity(0))
-    except AttributeError:
-        avail = os.cpu_count() or 1
+    avail = CPUS_AVAILABLE
     try:
         physical = int(info["Socket(s)"]) * int(info["Core(s) per socket"])
     except (KeyError, ValueError):

@@ -27,52 +27,56 @@ hor_ctx* ctx() {
   return g_ctx;
 }
 
-// FNV-1a over raw bytes.
-struct Hasher {
-  uint64_t h = 1469598103934665603ULL;
-  void add(const void* p, size_t n) {
-    const unsigned char* c = static_cast<const unsigned char*>(p);
-    for (size_t i = 0; i < n; i++) {
-      h ^= c[i];
-      h *= 1099511628211ULL;
-    }
-  }
-  template <class T>
-  void add(const std::vector<T>& v) { add(v.data(), v.size() * sizeof(T)); }
-  void add_d(double v) { add(&v, sizeof(v)); }
-  void add_i(int v) { add(&v, sizeof(v)); }
-};
-
 struct Flat {
   std::vector<double> sx, sn, sa, tx, tb, tx0, tn0;
 };
 
+// A cached plan and the exact geometry it was built from.  A call reuses the
+// plan only when its own geometry equals the stored copy value for value
+// (compared in place, straight from the caller's structures, without packing
+// or hashing); densities are uploaded on every call.
 struct CacheEntry {
-  uint64_t key;
-  hor_plan* plan;
+  int kind = 0, method = 0;
+  double DEL = 0;
+  double ctr[3] = {0, 0, 0};
+  hor_tree_params tree = {0, 0, 0};
+  Flat F;
+  hor_plan* plan = nullptr;
 };
 
 std::list<CacheEntry> g_cache;  // most recent first
 const size_t kCacheSize = 16;
 
-hor_plan* get_plan(int kind, int method, double DEL, const Flat& F, const double* ctr, uint64_t key) {
-  for (auto it = g_cache.begin(); it != g_cache.end(); ++it)
-    if (it->key == key) {
-      g_cache.splice(g_cache.begin(), g_cache, it);
-      return g_cache.front().plan;
-    }
-  int ns = (int)F.sa.size(), nt = (int)(F.tx.size() / 3);
-  hor_plan* p = nullptr;
-  check(hor_plan_create(ctx(), kind, method, HOR_FP64, DEL, ns, F.sx.data(), F.sn.data(), F.sa.data(), nt,
-                        F.tx.data(), F.tb.empty() ? nullptr : F.tb.data(), F.tx0.empty() ? nullptr : F.tx0.data(),
-                        F.tn0.empty() ? nullptr : F.tn0.data(), ctr, &g_tree, &p),
-        "hor_plan_create");
-  g_cache.push_front({key, p});
-  if (g_cache.size() > kCacheSize) {
-    hor_plan_destroy(g_cache.back().plan);
-    g_cache.pop_back();
+bool same_sources(const Flat& F, int N_quad, const vector<Surf_point>& S) {
+  if (F.sa.size() != (size_t)N_quad) return false;
+  for (int j = 0; j < N_quad; j++) {
+    const Surf_point& p = S[j];
+    if (F.sa[j] != p.Area) return false;
+    for (int k = 0; k < 3; k++)
+      if (F.sx[3 * j + k] != p.x[k] || F.sn[3 * j + k] != p.Nrml[k]) return false;
   }
-  return p;
+  return true;
+}
+
+bool same_targets(const Flat& F, int N_target, const vector<Target_point>& T, bool off) {
+  if (F.tx.size() != 3 * (size_t)N_target) return false;
+  for (int i = 0; i < N_target; i++) {
+    const Target_point& t = T[i];
+    if (F.tx[3 * i] != t.x || F.tx[3 * i + 1] != t.y || F.tx[3 * i + 2] != t.z) return false;
+    if (!off) continue;
+    if (F.tb[i] != t.b || F.tx0[3 * i] != t.nrst_x || F.tx0[3 * i + 1] != t.nrst_y || F.tx0[3 * i + 2] != t.nrst_z)
+      return false;
+    for (int k = 0; k < 3; k++)
+      if (F.tn0[3 * i + k] != t.nrst_Nrml[k]) return false;
+  }
+  return true;
+}
+
+bool same_setup(const CacheEntry& e, int kind, int method, double DEL, const double* ctr) {
+  if (e.kind != kind || e.method != method || e.DEL != DEL) return false;
+  if (method != HOR_TREE) return true;
+  return e.ctr[0] == ctr[0] && e.ctr[1] == ctr[1] && e.ctr[2] == ctr[2] && e.tree.P == g_tree.P &&
+         e.tree.N0 == g_tree.N0 && e.tree.theta == g_tree.theta;
 }
 
 void pack_sources(int N_quad, const vector<Surf_point>& S, Flat& F) {
@@ -95,10 +99,9 @@ std::vector<double> pack_density(int N_quad, const vector<Surf_point>& S, bool u
   return d;
 }
 
-// Packs targets; off-surface kinds also carry b, nearest point / normal and
-// d0 (nrst_f or nrst_g).
-void pack_targets(int N_target, const vector<Target_point>& T, bool off, int d0_kind, Flat& F,
-                  std::vector<double>& d0) {
+// Packs target positions; off-surface kinds also carry b and the nearest
+// point and normal.
+void pack_targets(int N_target, const vector<Target_point>& T, bool off, Flat& F) {
   F.tx.resize(3 * (size_t)N_target);
   for (int i = 0; i < N_target; i++) {
     F.tx[3 * i] = T[i].x;
@@ -108,43 +111,57 @@ void pack_targets(int N_target, const vector<Target_point>& T, bool off, int d0_
   F.tb.clear();
   F.tx0.clear();
   F.tn0.clear();
-  d0.clear();
   if (!off) return;
   F.tb.resize(N_target);
   F.tx0.resize(3 * (size_t)N_target);
   F.tn0.resize(3 * (size_t)N_target);
-  d0.resize(3 * (size_t)N_target);
   for (int i = 0; i < N_target; i++) {
     F.tb[i] = T[i].b;
     F.tx0[3 * i] = T[i].nrst_x;
     F.tx0[3 * i + 1] = T[i].nrst_y;
     F.tx0[3 * i + 2] = T[i].nrst_z;
-    for (int k = 0; k < 3; k++) {
-      F.tn0[3 * i + k] = T[i].nrst_Nrml[k];
-      d0[3 * i + k] = d0_kind == 0 ? T[i].nrst_f[k] : T[i].nrst_g[k];
-    }
+    for (int k = 0; k < 3; k++) F.tn0[3 * i + k] = T[i].nrst_Nrml[k];
   }
 }
 
-uint64_t geometry_key(int kind, int method, double DEL, const Flat& F, const double* ctr) {
-  Hasher H;
-  H.add_i(kind);
-  H.add_i(method);
-  H.add_d(DEL);
-  H.add(F.sx);
-  H.add(F.sn);
-  H.add(F.sa);
-  H.add(F.tx);
-  H.add(F.tb);
-  H.add(F.tx0);
-  H.add(F.tn0);
-  if (method == HOR_TREE) {
-    H.add(ctr, 3 * sizeof(double));
-    H.add_i(g_tree.P);
-    H.add_i(g_tree.N0);
-    H.add_d(g_tree.theta);
+// d0 of off-surface targets: nrst_f (SL) or nrst_g (DL).
+std::vector<double> pack_d0(int N_target, const vector<Target_point>& T, bool use_g) {
+  std::vector<double> d0(3 * (size_t)N_target);
+  for (int i = 0; i < N_target; i++)
+    for (int k = 0; k < 3; k++) d0[3 * i + k] = use_g ? T[i].nrst_g[k] : T[i].nrst_f[k];
+  return d0;
+}
+
+// The cached plan of this geometry, or a new one (the least recently used
+// entry is dropped beyond kCacheSize).
+hor_plan* plan_for(int kind, int method, double DEL, const double* ctr, int N_quad, const vector<Surf_point>& S,
+                   int N_target, const vector<Target_point>& T, bool off) {
+  for (auto it = g_cache.begin(); it != g_cache.end(); ++it)
+    if (same_setup(*it, kind, method, DEL, ctr) && same_sources(it->F, N_quad, S) &&
+        same_targets(it->F, N_target, T, off)) {
+      g_cache.splice(g_cache.begin(), g_cache, it);
+      return g_cache.front().plan;
+    }
+  CacheEntry e;
+  e.kind = kind;
+  e.method = method;
+  e.DEL = DEL;
+  if (ctr)
+    for (int k = 0; k < 3; k++) e.ctr[k] = ctr[k];
+  e.tree = g_tree;
+  pack_sources(N_quad, S, e.F);
+  pack_targets(N_target, T, off, e.F);
+  const Flat& F = e.F;
+  check(hor_plan_create(ctx(), kind, method, HOR_FP64, DEL, N_quad, F.sx.data(), F.sn.data(), F.sa.data(), N_target,
+                        F.tx.data(), F.tb.empty() ? nullptr : F.tb.data(), F.tx0.empty() ? nullptr : F.tx0.data(),
+                        F.tn0.empty() ? nullptr : F.tn0.data(), ctr, &g_tree, &e.plan),
+        "hor_plan_create");
+  g_cache.push_front(std::move(e));
+  if (g_cache.size() > kCacheSize) {
+    hor_plan_destroy(g_cache.back().plan);
+    g_cache.pop_back();
   }
-  return H.h;
+  return g_cache.front().plan;
 }
 
 // Batch evaluation: sources from Surfc (density f or g), targets from Target.
@@ -153,13 +170,11 @@ void batch(int kind, int method, int surf_num, double DEL, int N_quad, const vec
   std::lock_guard<std::mutex> lock(g_mu);
   bool dl = (kind == HOR_DL_ON || kind == HOR_DL_OFF);
   bool off = (kind == HOR_SL_OFF || kind == HOR_DL_OFF);
-  Flat F;
-  pack_sources(N_quad, Surfc, F);
-  std::vector<double> d0;
-  pack_targets(N_target, Target, off, dl ? 1 : 0, F, d0);
   const double* ctr = (method == HOR_TREE) ? g_ctr[surf_num == 1 ? 0 : 1] : nullptr;
-  hor_plan* p = get_plan(kind, method, DEL, F, ctr, geometry_key(kind, method, DEL, F, ctr));
+  hor_plan* p = plan_for(kind, method, DEL, ctr, N_quad, Surfc, N_target, Target, off);
   std::vector<double> dens = pack_density(N_quad, Surfc, dl);
+  std::vector<double> d0;
+  if (off) d0 = pack_d0(N_target, Target, dl);
   if (out.size() < 3 * (size_t)N_target) out.resize(3 * (size_t)N_target);
   check(hor_plan_eval(p, dens.data(), off ? d0.data() : nullptr, out.data()), "hor_plan_eval");
 }
@@ -169,13 +184,11 @@ void single(int kind, double DEL, int N_quad, const vector<Surf_point>& Surfc, c
             double x, double y, double z, double b, const double* x0, const double* n0, const double* d0,
             vector<double>& out) {
   std::lock_guard<std::mutex> lock(g_mu);
-  Flat F;
-  pack_sources(N_quad, Surfc, F);
-  F.tx = {x, y, z};
-  F.tb = {b};
-  F.tx0 = {x0[0], x0[1], x0[2]};
-  F.tn0 = {n0[0], n0[1], n0[2]};
-  hor_plan* p = get_plan(kind, HOR_DIRECT, DEL, F, nullptr, geometry_key(kind, HOR_DIRECT, DEL, F, nullptr));
+  vector<Target_point> T(1);
+  T[0].x = x; T[0].y = y; T[0].z = z; T[0].b = b;
+  T[0].nrst_x = x0[0]; T[0].nrst_y = x0[1]; T[0].nrst_z = x0[2];
+  for (int k = 0; k < 3; k++) T[0].nrst_Nrml[k] = n0[k];
+  hor_plan* p = plan_for(kind, HOR_DIRECT, DEL, nullptr, N_quad, Surfc, 1, T, true);
   if (dens.size() < 3 * (size_t)N_quad) throw std::runtime_error("density vector shorter than 3*N_quad");
   double r[3];
   check(hor_plan_eval(p, dens.data(), d0, r), "hor_plan_eval");

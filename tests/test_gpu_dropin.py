@@ -85,6 +85,12 @@ def _run_main(exe, cwd):
     v1 = np.loadtxt(os.path.join(cwd, "vel1_N32_highreg_57_k4"))
     v2 = np.loadtxt(os.path.join(cwd, "vel2_N32_highreg_57_k4"))
     t = re.search(r"CPU time elapsed: (\S+) seconds", p.stdout)
+    # per-call times the program prints: the four DL operators of each matvec (main.cpp:588-664)
+    # and each GMRES iteration (main.cpp:394-397)
+    dl = [float(x) for x in re.findall(r"Double layer evaluated \S+\s+with CPU time = (\S+)", p.stdout)]
+    its = [float(x) for x in re.findall(r"Iteration \d+ .*CPU time = (\S+) seconds", p.stdout)]
+    print(f"{os.path.basename(exe)}: {len(dl)} DL operator calls, mean {np.mean(dl) * 1e3:.2f} ms; "
+          f"{len(its)} iterations, mean {np.mean(its) * 1e3:.1f} ms", flush=True)
     return int(m.group(1)), float(m.group(2)), np.concatenate([v1.ravel(), v2.ravel()]), float(t.group(1))
 
 
