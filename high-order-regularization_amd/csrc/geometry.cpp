@@ -333,6 +333,91 @@ void traction_translating(const hor_body& B, const double U[3], int n, const dou
   }
 }
 
+// Exact Stokes flow of an ellipsoid with semi-axes a_k translating with
+// velocity U through fluid at rest at infinity (Oberbeck's solution), in the
+// body frame.  With lambda(x) >= 0 the ellipsoidal coordinate of the
+// confocal ellipsoid through x (sum_k x_k^2/(a_k^2 + lambda) = 1),
+// Delta(l) = sqrt(prod_k (a_k^2 + l)),
+//   chi(l)     = int_l^inf ds/Delta(s)                 = 2 R_F(a^2+l, b^2+l, c^2+l),
+//   alpha_j(l) = int_l^inf ds/((a_j^2+s) Delta(s))     = (2/3) R_D(.., .., a_j^2+l),
+// the velocity is the Papkovich-Neuber field v = grad(x.Phi + phi0) - 2 Phi
+// with the harmonic potentials Phi = sum_j C_j chi e_j and
+// phi0 = sum_j D_j dOmega/dx_j, Omega the exterior potential of the solid
+// ellipsoid (dOmega/dx_j = -2 x_j alpha_j), D_j = C_j a_j^2/2 (no tangential
+// slip on the surface) and C_j = -U_j/(chi(0) + a_j^2 alpha_j(0)) (v = U on it):
+//   v_j = -C_j (chi + a_j^2 alpha_j) - lambda lambda_j/Delta sum_i C_i x_i/(a_i^2 + lambda),
+//   lambda_j = dlambda/dx_j = (2 x_j/(a_j^2+lambda)) / sum_k x_k^2/(a_k^2+lambda)^2.
+// For a sphere it is Stokes' solution, for a prolate spheroid translating
+// along its axis SL_exact (extend_to_grid/utilities.cpp:360-411); its drag
+// is F_j = -16 pi mu C_j = 16 pi mu U_j/(chi(0) + a_j^2 alpha_j(0)), the
+// force of traction_translating's density.  Inside and on the body v = U.
+void ellipsoid_exact(const hor_body& B, const double U[3], int n, const double* xin, double* uout) {
+  const double a2[3] = {B.abc[0] * B.abc[0], B.abc[1] * B.abc[1], B.abc[2] * B.abc[2]};
+  double Ub[3];
+  if (B.rotated) {
+    for (int i = 0; i < 3; i++) Ub[i] = B.R[3 * i] * U[0] + B.R[3 * i + 1] * U[1] + B.R[3 * i + 2] * U[2];
+  } else {
+    Ub[0] = U[0]; Ub[1] = U[1]; Ub[2] = U[2];
+  }
+  const double chi0 = 2.0 * carlson_rf(a2[0], a2[1], a2[2]);
+  double Cc[3];
+  for (int j = 0; j < 3; j++) {
+    const double al0 = 2.0 / 3.0 * carlson_rd(a2[(j + 1) % 3], a2[(j + 2) % 3], a2[j]);
+    Cc[j] = -Ub[j] / (chi0 + a2[j] * al0);
+  }
+#pragma omp parallel for schedule(static)
+  for (int p = 0; p < n; p++) {
+    double x[3];
+    to_body(B, xin + 3 * p, true, x);
+    double v[3];
+    const double q0 = x[0] * x[0] / a2[0] + x[1] * x[1] / a2[1] + x[2] * x[2] / a2[2];
+    if (!(q0 > 1.0)) {
+      for (int j = 0; j < 3; j++) v[j] = Ub[j];
+    } else {
+      // lambda: the root of F(l) = sum x_k^2/(a_k^2+l) - 1 (decreasing) in (0, |x|^2]:
+      // Newton from the upper end, safeguarded by bisection
+      double lo = 0.0, hi = x[0] * x[0] + x[1] * x[1] + x[2] * x[2], l = hi;
+      for (int it = 0; it < 200; it++) {
+        double F = -1.0, dF = 0.0;
+        for (int k = 0; k < 3; k++) {
+          const double d = 1.0 / (a2[k] + l);
+          F += x[k] * x[k] * d;
+          dF -= x[k] * x[k] * d * d;
+        }
+        if (F > 0.0) lo = l;
+        else hi = l;
+        double ln = l - F / dF;
+        if (!(ln > lo && ln < hi)) ln = 0.5 * (lo + hi);
+        if (std::fabs(ln - l) <= 1e-16 * std::max(l, 1e-300) || hi - lo <= 1e-16 * hi) {
+          l = ln;
+          break;
+        }
+        l = ln;
+      }
+      double e[3], S2 = 0.0, Delta = 1.0;
+      for (int k = 0; k < 3; k++) {
+        e[k] = a2[k] + l;
+        S2 += x[k] * x[k] / (e[k] * e[k]);
+        Delta *= e[k];
+      }
+      Delta = std::sqrt(Delta);
+      const double chi = 2.0 * carlson_rf(e[0], e[1], e[2]);
+      double sum = 0.0;
+      for (int i = 0; i < 3; i++) sum += Cc[i] * x[i] / e[i];
+      for (int j = 0; j < 3; j++) {
+        const double alj = 2.0 / 3.0 * carlson_rd(e[(j + 1) % 3], e[(j + 2) % 3], e[j]);
+        const double lj = 2.0 * x[j] / e[j] / S2;
+        v[j] = -Cc[j] * (chi + a2[j] * alj) - l * lj / Delta * sum;
+      }
+    }
+    if (B.rotated) {
+      for (int i = 0; i < 3; i++) uout[3 * p + i] = B.R[i] * v[0] + B.R[3 + i] * v[1] + B.R[6 + i] * v[2];
+    } else {
+      for (int i = 0; i < 3; i++) uout[3 * p + i] = v[i];
+    }
+  }
+}
+
 // SL_exact (extend_to_grid/utilities.cpp:360-411): exterior velocity of a
 // prolate spheroid (semi-axes a > b = c along body x) translating with
 // U = (1,0,0); inside and on the body the velocity is U.
@@ -486,6 +571,13 @@ int hor_level_set(const hor_body* body, int n, const double* x, double* phi_out)
   if (!body || !x || !phi_out || n < 0) return hor::set_error(HOR_E_ARG, "hor_level_set: bad argument");
 #pragma omp parallel for schedule(static)
   for (int i = 0; i < n; i++) phi_out[i] = hor::geom::phi(*body, x + 3 * i, true);
+  return HOR_OK;
+}
+
+int hor_translating_exact_velocity(const hor_body* body, const double* U, int n, const double* x, double* u) {
+  if (!body || !U || !x || !u || n < 0)
+    return hor::set_error(HOR_E_ARG, "hor_translating_exact_velocity: bad argument");
+  hor::ellipsoid_exact(*body, U, n, x, u);
   return HOR_OK;
 }
 

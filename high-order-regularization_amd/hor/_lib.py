@@ -116,6 +116,7 @@ _SIGS = {
     "hor_stencil_build": (_I, [_P, _P, _D, _I, _P, _I, _P, _P, _P, _P, _P]),
     "hor_traction_translating": (_I, [C.POINTER(Body), _P, _I, _P, _P]),
     "hor_spheroid_exact_velocity": (_I, [C.POINTER(Body), _I, _P, _P]),
+    "hor_translating_exact_velocity": (_I, [C.POINTER(Body), _P, _I, _P, _P]),
 }
 
 EXPORTED = sorted(_SIGS)

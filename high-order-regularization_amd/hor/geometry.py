@@ -122,6 +122,15 @@ class Body:
               "hor_traction_translating")
         return f
 
+    def exact_velocity(self, x, U=(1.0, 0.0, 0.0)):
+        """Exact velocity of this ellipsoid translating with U (U inside and on
+        the body, Oberbeck's exterior flow outside)."""
+        x, u0 = f64(x), f64(np.asarray(U), 3)
+        u = np.zeros_like(x)
+        check(_lib.lib().hor_translating_exact_velocity(C.byref(self.c), ptr(u0), x.size // 3, ptr(x), ptr(u)),
+              "hor_translating_exact_velocity")
+        return u
+
     def spheroid_exact(self, x):
         """Exact exterior velocity of the prolate spheroid translating with U=(1,0,0) in its body frame."""
         x = f64(x)

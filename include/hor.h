@@ -479,6 +479,15 @@ int hor_traction_translating(const hor_body* body, const double* U, int n,
                              const double* x, double* f);
 int hor_spheroid_exact_velocity(const hor_body* body, int n, const double* x,
                                 double* u);
+/* Exact velocity of any ellipsoid body (semi-axes abc, rotation R) translating
+ * with velocity U through fluid at rest at infinity (Oberbeck's solution in
+ * ellipsoidal integrals, Carlson R_F / R_D): U inside and on the body, the
+ * exterior Stokes flow outside.  For a prolate spheroid translating along its
+ * axis it is SL_exact (extend_to_grid/utilities.cpp:360-411); its force is
+ * that of hor_traction_translating's density.  No reference counterpart for
+ * other bodies. */
+int hor_translating_exact_velocity(const hor_body* body, const double* U, int n,
+                                   const double* x, double* u);
 /* p_exact (extend_to_grid/utilities.cpp:294-321): pressure of that spheroid. */
 int hor_spheroid_exact_pressure(const hor_body* body, int n, const double* x,
                                 double* p);

@@ -7,7 +7,8 @@ DL_OFF and the pressure over every node.
 * size-independent properties over all nodes: linearity in the density
   (2 f1 - 3 f2 against 2 SL[f1] - 3 SL[f2], 1e-12), and for the translating
   body's traction the single layer is U = (1,0,0) at every node inside the
-  body (max error 1e-6 at this h).
+  body (max error 1e-6 at this h) and Oberbeck's exact flow at every node
+  outside (max error 1e-4 at this h).
 """
 import numpy as np
 import pytest
@@ -68,11 +69,16 @@ def test_full_size_linearity_and_rigid_motion(ctx, band64):
     u1, u2 = P(f1, f01), P(f2, f02)
     u3 = P(2 * f1 - 3 * f2, 2 * f01 - 3 * f02)
     assert rel_max(u3, 2 * u1 - 3 * u2) <= 1e-12
-    inside = t["b"] < 0
-    assert inside.sum() > 100000
+    inside, outside = t["b"] < 0, t["b"] > 0
+    assert inside.sum() > 100000 and outside.sum() > 100000
     err = np.abs(u1.reshape(-1, 3)[inside] - np.array([1.0, 0.0, 0.0])).max()
     print("SL of the translating traction inside the body: max |u - U| =", err)
     assert err < 1e-6
+    # outside: Oberbeck's exact flow of the translating ellipsoid
+    ex = body.exact_velocity(t["x"]).reshape(-1, 3)
+    e_out = np.abs(u1.reshape(-1, 3)[outside] - ex[outside]).max()
+    print(f"outside the body ({outside.sum()} nodes, |b| <= 0.3): max |u - exact| = {e_out:.3e}")
+    assert e_out < 1e-4
     P.close()
 
 
