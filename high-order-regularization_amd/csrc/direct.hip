@@ -37,7 +37,10 @@ __global__ __launch_bounds__(kBlock) void k_direct(DirectArgs A) {
   // this chunk's 64-source subtiles
   const int sub0 = blockIdx.y * A.subs_per_chunk;
   const int sub1 = min(A.nsub, sub0 + A.subs_per_chunk);
-  range_sum<KIND, MIXED, true>(A, sub0 * kWave, min(A.ns, sub1 * kWave), sub0, wb, all_far, t, tab, u);
+  if constexpr (KIND == HOR_P_OFF)
+    range_sum_p(A, sub0 * kWave, min(A.ns, sub1 * kWave), sub0, wb, all_far, t, tab, u);
+  else
+    range_sum<KIND, MIXED, true>(A, sub0 * kWave, min(A.ns, sub1 * kWave), sub0, wb, all_far, t, tab, u);
   double* P = A.partial + (size_t)blockIdx.y * 3 * A.ntp;
   P[i] = u[0];
   if (KIND != HOR_P_OFF) {
@@ -167,6 +170,28 @@ __global__ void k_gather3(const double* dens, const int* perm, int n, int np, co
     q.d0 = (float)r.d0; q.d1 = (float)r.d1; q.d2 = (float)r.d2; q.pad1 = 0.0f;
     rec32[j] = q;
   }
+}
+
+// Pressure records: recp[j] <- (y, a n.y', a n, a f.y', a f, a n x y') of
+// sorted source j with y' = y - c0, dens in the caller's order.
+__global__ void k_gather_p(const double* dens, const int* perm, int n, int np, const double* sx, const double* sn,
+                           const double* sa, double c0x, double c0y, double c0z, SrcRecP* recp) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  int o = perm[j];
+  const double a = sa[j];
+  SrcRecP r;
+  r.x = sx[j]; r.y = sx[np + j]; r.z = sx[2 * np + j];
+  r.nx = a * sn[j]; r.ny = a * sn[np + j]; r.nz = a * sn[2 * np + j];
+  r.d0 = a * dens[3 * o]; r.d1 = a * dens[3 * o + 1]; r.d2 = a * dens[3 * o + 2];
+  const double yx = r.x - c0x, yy = r.y - c0y, yz = r.z - c0z;
+  r.al = r.nx * yx + r.ny * yy + r.nz * yz;
+  r.be = r.d0 * yx + r.d1 * yy + r.d2 * yz;
+  r.m0 = r.ny * yz - r.nz * yy;
+  r.m1 = r.nz * yx - r.nx * yz;
+  r.m2 = r.nx * yy - r.ny * yx;
+  r.pad0 = r.pad1 = 0.0;
+  recp[j] = r;
 }
 
 // Fused records (HOR_SLDL_OFF), dens = [f | g]: recv[j] <- (y, a n, a f, g)
@@ -316,6 +341,15 @@ int launch_gather_sldl(const double* dens, const int* perm, int n, int np, const
   if (n == 0) return HOR_OK;
   hipLaunchKernelGGL(k_gather_sldl, dim3((n + 255) / 256), dim3(256), 0, s, dens, perm, n, np, sx, sn, sa, recv,
                      recv32, sub_box);
+  HOR_HIP_TRY(hipGetLastError());
+  return HOR_OK;
+}
+
+int launch_gather_p(const double* dens, const int* perm, int n, int np, const double* sx, const double* sn,
+                    const double* sa, const double c0[3], SrcRecP* recp, hipStream_t s) {
+  if (n == 0) return HOR_OK;
+  hipLaunchKernelGGL(k_gather_p, dim3((n + 255) / 256), dim3(256), 0, s, dens, perm, n, np, sx, sn, sa, c0[0], c0[1],
+                     c0[2], recp);
   HOR_HIP_TRY(hipGetLastError());
   return HOR_OK;
 }

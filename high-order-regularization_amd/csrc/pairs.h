@@ -567,7 +567,8 @@ __device__ __forceinline__ void dl_on_reg32(const SrcRec32& s, float dx, float d
 // every product then reads at most one source value (the wave holds the
 // record in scalar registers and a VALU instruction reads at most one), so
 // no record value is first copied to vector registers.
-__device__ __forceinline__ double p_numer(const SrcRec& s, const Tgt& t, double dx, double dy, double dz) {
+template <class Rec>
+__device__ __forceinline__ double p_numer(const Rec& s, const Tgt& t, double dx, double dy, double dz) {
   const double nd = s.nx * dx + s.ny * dy + s.nz * dz;
   const double fd = s.d0 * dx + s.d1 * dy + s.d2 * dz;
   const double cx = dy * t.g0z - dz * t.g0y, cy = dz * t.g0x - dx * t.g0z, cz = dx * t.g0y - dy * t.g0x;
@@ -575,18 +576,49 @@ __device__ __forceinline__ double p_numer(const SrcRec& s, const Tgt& t, double 
   return (t.sub * nd - fd) + nc;
 }
 
-// Plain kernel H2 = 1/r^3 (Lapl_DL_7ord beyond 8 delta).
-__device__ __forceinline__ void p_plain(const SrcRec& s, const Tgt& t, double u[3]) {
+// Per-target data of the pressure's plain pairs: p' = p - c0 and
+// q = (f0.n0) p' + p' x (n0 x f0).
+struct TgtP {
+  double px, py, pz, qx, qy, qz;
+};
+
+__device__ __forceinline__ void make_tgtp(const Tgt& t, const double c0[3], TgtP& v) {
+  v.px = t.px - c0[0];
+  v.py = t.py - c0[1];
+  v.pz = t.pz - c0[2];
+  v.qx = t.sub * v.px + (v.py * t.g0z - v.pz * t.g0y);
+  v.qy = t.sub * v.py + (v.pz * t.g0x - v.px * t.g0z);
+  v.qz = t.sub * v.pz + (v.px * t.g0y - v.py * t.g0x);
+}
+
+// Plain kernel H2 = 1/r^3 (Lapl_DL_7ord beyond 8 delta).  With d = p' - x'
+// (x' = x - c0) the numerator (f0.n0)(n.d) - f.d + n.(d x c), c = n0 x f0,
+// is n.q - f.p' - c.m + (be - (f0.n0) al): one FMA chain on the record's
+// precomputed al, be, m.  Its terms are of size |p'| where the numerator is
+// of size |d| > 8 delta, so the chain's rounding stays within a few ulps of
+// the numerator at the distances of plain pairs.
+__device__ __forceinline__ void p_plain(const SrcRecP& s, const Tgt& t, const TgtP& v, double u[3]) {
   const double dx = t.px - s.x, dy = t.py - s.y, dz = t.pz - s.z;
   const double r2 = dx * dx + dy * dy + dz * dz;
   const double ri = rsqrt64(r2);
-  u[0] += p_numer(s, t, dx, dy, dz) * (ri * ri * ri);
+  double a = v.qx * s.nx;
+  a = __builtin_fma(v.qy, s.ny, a);
+  a = __builtin_fma(v.qz, s.nz, a);
+  a = __builtin_fma(-v.px, s.d0, a);
+  a = __builtin_fma(-v.py, s.d1, a);
+  a = __builtin_fma(-v.pz, s.d2, a);
+  a = __builtin_fma(-t.g0x, s.m0, a);
+  a = __builtin_fma(-t.g0y, s.m1, a);
+  a = __builtin_fma(-t.g0z, s.m2, a);
+  a = __builtin_fma(-t.sub, s.al, a);
+  a = a + s.be;
+  u[0] += a * (ri * ri * ri);
 }
 
 // Full branch structure of Lapl_DL_7ord (extend_to_grid/utilities.cpp:190-211):
 // self limit, plain, or (erf + P2(rho^2) rho e^{-rho^2}) / r^3 with the
 // Stokes_SL_7ord s2 polynomial (the same coefficients q[3..6]).
-__device__ __forceinline__ void p_full(const SrcRec& s, const Tgt& t, double inv_del, double inv_del2,
+__device__ __forceinline__ void p_full(const SrcRecP& s, const Tgt& t, double inv_del, double inv_del2,
                                        double self2, const double* __restrict__ tab, double u[3]) {
   const double dx = t.px - s.x, dy = t.py - s.y, dz = t.pz - s.z;
   const double r2 = dx * dx + dy * dy + dz * dz;
@@ -667,7 +699,6 @@ __device__ __forceinline__ void range_sum(const DirectArgs& A, int j0, int j1, i
   const SrcRec* __restrict__ R = A.rec;
   const SrcRec32* __restrict__ R32 = A.rec32;
   constexpr bool SL = (KIND == HOR_SL_ON || KIND == HOR_SL_OFF);
-  constexpr bool PR = (KIND == HOR_P_OFF);   // FP64 only
   Tgt32 v;
   if (MIXED) make_tgt32(t, v);
   const float inv_del32 = (float)inv_del;
@@ -693,8 +724,7 @@ __device__ __forceinline__ void range_sum(const DirectArgs& A, int j0, int j1, i
       } else {
 #pragma unroll 2
         for (int j = base; j < end; j++) {
-          if (PR) p_plain(R[j], t, u);
-          else if (SL) sl_plain(R[j], t.px, t.py, t.pz, t.sub, u);
+          if (SL) sl_plain(R[j], t.px, t.py, t.pz, t.sub, u);
           else dl_plain(R[j], t.px, t.py, t.pz, t.g0x, t.g0y, t.g0z, u);
         }
       }
@@ -716,13 +746,11 @@ __device__ __forceinline__ void range_sum(const DirectArgs& A, int j0, int j1, i
           else if (KIND == HOR_DL_OFF) dl_off_reg32(s32, fx, fy, fz, v, t.nearp, inv_del32, w);
           else dl_on_reg32(s32, fx, fy, fz, v, inv_del32, w);
         } else if (__any(need ? 1 : 0)) {
-          if (PR) p_full(s, t, inv_del, inv_del2, self2, tab, u);
-          else if (SL) sl_full(s, t.px, t.py, t.pz, t.sub, t.nearp, t.q, inv_del, inv_del2, self1, self2, tab, u);
+          if (SL) sl_full(s, t.px, t.py, t.pz, t.sub, t.nearp, t.q, inv_del, inv_del2, self1, self2, tab, u);
           else if (KIND == HOR_DL_OFF) dl_off_full(s, t, inv_del, inv_del2, self2, tab, u);
           else dl_on_full(s, t, inv_del2, inv_del5, tab, u);
         } else {
-          if (PR) p_plain(s, t, u);
-          else if (SL) sl_plain(s, t.px, t.py, t.pz, t.sub, u);
+          if (SL) sl_plain(s, t.px, t.py, t.pz, t.sub, u);
           else dl_plain(s, t.px, t.py, t.pz, t.g0x, t.g0y, t.g0z, u);
         }
       }
@@ -730,6 +758,37 @@ __device__ __forceinline__ void range_sum(const DirectArgs& A, int j0, int j1, i
         u[0] += (double)w[0];
         u[1] += (double)w[1];
         u[2] += (double)w[2];
+      }
+    }
+  }
+}
+
+// Regularized pressure over the sorted sources [j0, j1) (HOR_P_OFF, FP64):
+// the subtile test and per-source vote of range_sum on the pressure records.
+__device__ __forceinline__ void range_sum_p(const DirectArgs& A, int j0, int j1, int sub0, const double* wb,
+                                            bool all_far, const Tgt& t, const double* __restrict__ tab,
+                                            double u[3]) {
+  const double rp = 1.7724538509055160273;
+  const double inv_del = 1.0 / A.DEL;
+  const double inv_del2 = inv_del * inv_del;
+  const double self2 = 256.0 / 15.0 / rp / (A.DEL * A.DEL * A.DEL);
+  const SrcRecP* __restrict__ R = A.recp;
+  TgtP v;
+  make_tgtp(t, A.c0, v);
+  for (int base = j0, sb = sub0; base < j1; base += kWave, sb++) {
+    const int end = min(j1, base + kWave);
+    const double* box = A.sub_box + 6 * sb;
+    if (all_far || boxes_far(wb, box, A.far_d2)) {
+#pragma unroll 2
+      for (int j = base; j < end; j++) p_plain(R[j], t, v, u);
+    } else {
+      for (int j = base; j < end; j++) {
+        const SrcRecP s = R[j];
+        const double dx = t.px - s.x, dy = t.py - s.y, dz = t.pz - s.z;
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        const bool need = t.nearp && !(r2 > A.near_r2);
+        if (__any(need ? 1 : 0)) p_full(s, t, inv_del, inv_del2, self2, tab, u);
+        else p_plain(s, t, v, u);
       }
     }
   }

@@ -263,6 +263,15 @@ int setup_sources(hor_plan* P, const std::vector<int>& perm, const double* sx, c
   if (P->kind == HOR_SLDL_OFF) {
     HOR_TRY(P->recv.alloc(nsp));
     if (P->precision == HOR_MIXED) HOR_TRY(P->recv32.alloc(nsp));
+  } else if (P->kind == HOR_P_OFF) {
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int j = 0; j < ns; j++)
+      for (int k = 0; k < 3; k++) {
+        lo[k] = std::min(lo[k], sx[3 * j + k]);
+        hi[k] = std::max(hi[k], sx[3 * j + k]);
+      }
+    for (int k = 0; k < 3; k++) P->c0[k] = 0.5 * (lo[k] + hi[k]);
+    HOR_TRY(P->recp.alloc(nsp));
   } else {
     HOR_TRY(P->rec.alloc(nsp));
     if (P->precision == HOR_MIXED) HOR_TRY(P->rec32.alloc(nsp));
@@ -284,7 +293,8 @@ DirectArgs direct_args(hor_plan* P) {
   A.near_r2 = 64.0 * P->DEL * P->DEL * (1.0 + 1e-12);
   A.near64_r2 = 16.0 * P->DEL * P->DEL;
   A.sx = P->sx.p; A.sn = P->sn.p; A.sa = P->sa.p; A.sd = P->sd.p; A.sub_box = P->sub_box.p;
-  A.rec = P->rec.p; A.recv = P->recv.p; A.rec32 = P->precision == HOR_MIXED ? P->rec32.p : nullptr;
+  A.rec = P->rec.p; A.recv = P->recv.p; A.recp = P->recp.p;
+  for (int k = 0; k < 3; k++) A.c0[k] = P->c0[k]; A.rec32 = P->precision == HOR_MIXED ? P->rec32.p : nullptr;
   A.recv32 = P->precision == HOR_MIXED ? P->recv32.p : nullptr;
   A.tx = P->tx.p; A.tcoef = P->tcoef.p; A.tsub = P->tsub.p;
   A.tx0 = P->tx0.p; A.tn0 = P->tn0.p; A.tb = P->tb.p;
@@ -354,9 +364,12 @@ static int plan_eval_direct(hor_plan* P, const double* dens_dev, const double* d
     return launch_sldl_finalize(P->partial.p, P->nchunk, P->nt, P->ntp, op_scale(HOR_SL_OFF), op_scale(HOR_DL_OFF),
                                 P->xi.p, P->tsub.p + P->ntp, P->tperm.p, out_dev, s);
   }
-  HOR_TRY(launch_gather3(dens_dev, P->sperm.p, P->ns, P->nsp, P->sx.p, P->sn.p, P->sa.p, P->sd.p, P->rec.p,
-                         P->precision == HOR_MIXED ? P->rec32.p : nullptr, P->sub_box.p,
-                         P->kind == HOR_SL_ON || P->kind == HOR_SL_OFF || P->kind == HOR_P_OFF, s));
+  if (P->kind == HOR_P_OFF)
+    HOR_TRY(launch_gather_p(dens_dev, P->sperm.p, P->ns, P->nsp, P->sx.p, P->sn.p, P->sa.p, P->c0, P->recp.p, s));
+  else
+    HOR_TRY(launch_gather3(dens_dev, P->sperm.p, P->ns, P->nsp, P->sx.p, P->sn.p, P->sa.p, P->sd.p, P->rec.p,
+                           P->precision == HOR_MIXED ? P->rec32.p : nullptr, P->sub_box.p,
+                           P->kind == HOR_SL_ON || P->kind == HOR_SL_OFF, s));
   HOR_TRY(prepare_targets(P, dens_dev, d0_dev));
   DirectArgs A = direct_args(P);
   HOR_HIP_TRY(hipEventRecord(c->ev[2], s));
