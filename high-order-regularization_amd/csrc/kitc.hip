@@ -292,80 +292,138 @@ constexpr int kNc32Dl = 12;   // (9 used)
 // d.M.d uses only the symmetric part of M, so the moments hold its diagonal
 // and the three off-diagonal sums: 9 components with W_n).
 // One work item is a far node and up to 256 of its targets; the node's
-// proxies and moments are wave-uniform.  In mixed precision the proxy
-// interactions run in FP32 on node-centred coordinates, each row of P+1
-// proxies is summed in FP32 and the rows in FP64.
-template <bool DL, bool MIXED>
-__global__ __launch_bounds__(kBlock) void k_far(FarArgs A) {
+// proxies and moments are wave-uniform (scalar loads).  Each lane carries T
+// targets (target tid + q * 256/T of the item for q < T): every moment loaded
+// serves T targets and the T chains of arithmetic interleave.  In mixed
+// precision the proxy interactions run in FP32 on node-centred coordinates,
+// each row of P+1 proxies is summed in FP32 and the rows in FP64.
+template <bool DL, bool MIXED, int T>
+__global__ __launch_bounds__(kBlock / T) void k_far(FarArgs A) {
+  constexpr int NT = kBlock / T;           // threads per workgroup
   const int tid = threadIdx.x;
   const int item = blockIdx.x;
   const int node = A.item_node[item];
   const int off = A.item_off[item], cnt = A.item_cnt[item];
   if ((tid & ~(kWave - 1)) >= cnt) return;  // a wave without targets (no workgroup barrier here)
-  const bool act = tid < cnt;
-  const int li = off + (act ? tid : 0);
-  const int i = A.tgt[li];
+  bool act[T];
+  int li[T], ti[T];
+#pragma unroll
+  for (int q = 0; q < T; q++) {
+    const int k = tid + q * NT;
+    act[q] = k < cnt;
+    li[q] = off + (act[q] ? k : 0);
+    ti[q] = A.tgt[li[q]];
+  }
   const int np1 = A.np1, pflat = A.pflat;
-  const double px = A.tx[i], py = A.tx[A.ntp + i], pz = A.tx[2 * A.ntp + i];
+  double px[T], py[T], pz[T];
+#pragma unroll
+  for (int q = 0; q < T; q++) {
+    px[q] = A.tx[ti[q]];
+    py[q] = A.tx[A.ntp + ti[q]];
+    pz[q] = A.tx[2 * A.ntp + ti[q]];
+  }
   const int fi = A.far_idx[node];
-  double u0 = 0.0, u1 = 0.0, u2 = 0.0;
+  double u0[T], u1[T], u2[T];
+#pragma unroll
+  for (int q = 0; q < T; q++) u0[q] = u1[q] = u2[q] = 0.0;
   if (MIXED) {
     const float* t1 = A.cheb32 + (size_t)3 * np1 * node;
     const float* t2 = t1 + np1;
     const float* t3 = t2 + np1;
     const double* ctr = A.node_ctr + 3 * node;
-    const float qx = (float)(px - ctr[0]), qy = (float)(py - ctr[1]), qz = (float)(pz - ctr[2]);
+    float qx[T], qy[T], qz[T];
+#pragma unroll
+    for (int q = 0; q < T; q++) {
+      qx[q] = (float)(px[q] - ctr[0]);
+      qy[q] = (float)(py[q] - ctr[1]);
+      qz[q] = (float)(pz[q] - ctr[2]);
+    }
     constexpr int NC = DL ? kNc32Dl : kNc32Sl;
     const float* M = A.mom32 + (size_t)fi * pflat * NC;
     if (!DL) {
-      const float sub = (float)A.tsub2[i];
+      float sub[T];
+#pragma unroll
+      for (int q = 0; q < T; q++) sub[q] = (float)A.tsub2[ti[q]];
       for (int a = 0, kk = 0; a < np1; a++) {
-        const float d0 = qx - t1[a];
+        float d0[T];
+#pragma unroll
+        for (int q = 0; q < T; q++) d0[q] = qx[q] - t1[a];
         for (int b = 0; b < np1; b++) {
-          const float d1 = qy - t2[b];
-          const float r2ab = d0 * d0 + d1 * d1;
-          float v0 = 0.f, v1 = 0.f, v2 = 0.f;
+          float d1[T], r2ab[T], v0[T], v1[T], v2[T];
+#pragma unroll
+          for (int q = 0; q < T; q++) {
+            d1[q] = qy[q] - t2[b];
+            r2ab[q] = d0[q] * d0[q] + d1[q] * d1[q];
+            v0[q] = v1[q] = v2[q] = 0.f;
+          }
 #pragma unroll 4
           for (int c = 0; c < np1; c++, kk++) {
             const float* m = M + (size_t)kk * NC;
-            const float d2 = qz - t3[c];
-            const float ri = rsqrt32(__builtin_fmaf(d2, d2, r2ab));
-            const float m0 = m[0] - sub * m[3], m1 = m[1] - sub * m[4], m2 = m[2] - sub * m[5];
-            const float tq = (m0 * d0 + m1 * d1 + m2 * d2) * (ri * ri);
-            v0 += (m0 + tq * d0) * ri;
-            v1 += (m1 + tq * d1) * ri;
-            v2 += (m2 + tq * d2) * ri;
+#pragma unroll
+            for (int q = 0; q < T; q++) {
+              const float d2 = qz[q] - t3[c];
+              const float ri = rsqrt32(__builtin_fmaf(d2, d2, r2ab[q]));
+              const float m0 = m[0] - sub[q] * m[3], m1 = m[1] - sub[q] * m[4], m2 = m[2] - sub[q] * m[5];
+              const float tq = (m0 * d0[q] + m1 * d1[q] + m2 * d2) * (ri * ri);
+              v0[q] += (m0 + tq * d0[q]) * ri;
+              v1[q] += (m1 + tq * d1[q]) * ri;
+              v2[q] += (m2 + tq * d2) * ri;
+            }
           }
-          u0 += (double)v0; u1 += (double)v1; u2 += (double)v2;
+#pragma unroll
+          for (int q = 0; q < T; q++) {
+            u0[q] += (double)v0[q]; u1[q] += (double)v1[q]; u2[q] += (double)v2[q];
+          }
         }
       }
     } else {
-      const float g0x = (float)A.tsub2[i], g0y = (float)A.tsub2[A.ntp + i], g0z = (float)A.tsub2[2 * A.ntp + i];
+      float g0x[T], g0y[T], g0z[T];
+#pragma unroll
+      for (int q = 0; q < T; q++) {
+        g0x[q] = (float)A.tsub2[ti[q]];
+        g0y[q] = (float)A.tsub2[A.ntp + ti[q]];
+        g0z[q] = (float)A.tsub2[2 * A.ntp + ti[q]];
+      }
       for (int a = 0, kk = 0; a < np1; a++) {
-        const float d0 = qx - t1[a];
-        const float d00 = d0 * d0;
+        float d0[T], d00[T];
+#pragma unroll
+        for (int q = 0; q < T; q++) {
+          d0[q] = qx[q] - t1[a];
+          d00[q] = d0[q] * d0[q];
+        }
         for (int b = 0; b < np1; b++) {
-          const float d1 = qy - t2[b];
-          const float r2ab = __builtin_fmaf(d1, d1, d00);
-          const float gab = g0x * d0 + g0y * d1;
-          const float d11 = d1 * d1, d01 = d0 * d1;
-          float kab = 0.f, v2 = 0.f;
+          float d1[T], r2ab[T], gab[T], d11[T], d01[T], kab[T], v2[T];
+#pragma unroll
+          for (int q = 0; q < T; q++) {
+            d1[q] = qy[q] - t2[b];
+            r2ab[q] = __builtin_fmaf(d1[q], d1[q], d00[q]);
+            gab[q] = g0x[q] * d0[q] + g0y[q] * d1[q];
+            d11[q] = d1[q] * d1[q];
+            d01[q] = d0[q] * d1[q];
+            kab[q] = v2[q] = 0.f;
+          }
 #pragma unroll 4
           for (int c = 0; c < np1; c++, kk++) {
             const float* m = M + (size_t)kk * NC;
-            const float d2 = qz - t3[c];
-            const float ri = rsqrt32(__builtin_fmaf(d2, d2, r2ab));
-            const float ri2 = ri * ri;
-            const float qab = __builtin_fmaf(m[3], d01, __builtin_fmaf(m[1], d11, m[0] * d00));
-            const float w = __builtin_fmaf(m[2], d2, __builtin_fmaf(m[5], d1, m[4] * d0));
-            const float wnd = m[6] * d0 + m[7] * d1 + m[8] * d2;
-            const float gd = __builtin_fmaf(g0z, d2, gab);
-            const float Q = __builtin_fmaf(-gd, wnd, __builtin_fmaf(w, d2, qab));
-            const float kern = Q * (ri2 * ri2 * ri);
-            kab += kern;
-            v2 += d2 * kern;
+#pragma unroll
+            for (int q = 0; q < T; q++) {
+              const float d2 = qz[q] - t3[c];
+              const float ri = rsqrt32(__builtin_fmaf(d2, d2, r2ab[q]));
+              const float ri2 = ri * ri;
+              const float qab = __builtin_fmaf(m[3], d01[q], __builtin_fmaf(m[1], d11[q], m[0] * d00[q]));
+              const float w = __builtin_fmaf(m[2], d2, __builtin_fmaf(m[5], d1[q], m[4] * d0[q]));
+              const float wnd = m[6] * d0[q] + m[7] * d1[q] + m[8] * d2;
+              const float gd = __builtin_fmaf(g0z[q], d2, gab[q]);
+              const float Q = __builtin_fmaf(-gd, wnd, __builtin_fmaf(w, d2, qab));
+              const float kern = Q * (ri2 * ri2 * ri);
+              kab[q] += kern;
+              v2[q] += d2 * kern;
+            }
           }
-          u0 += (double)(d0 * kab); u1 += (double)(d1 * kab); u2 += (double)v2;
+#pragma unroll
+          for (int q = 0; q < T; q++) {
+            u0[q] += (double)(d0[q] * kab[q]); u1[q] += (double)(d1[q] * kab[q]); u2[q] += (double)v2[q];
+          }
         }
       }
     }
@@ -376,65 +434,105 @@ __global__ __launch_bounds__(kBlock) void k_far(FarArgs A) {
     constexpr int NC = DL ? 9 : 6;
     const double* M = A.mom + (size_t)fi * pflat * NC;
     if (!DL) {
-      const double sub = A.tsub2[i];
+      double sub[T];
+#pragma unroll
+      for (int q = 0; q < T; q++) sub[q] = A.tsub2[ti[q]];
       for (int a = 0, kk = 0; a < np1; a++) {
-        const double d0 = px - t1[a];
+        double d0[T];
+#pragma unroll
+        for (int q = 0; q < T; q++) d0[q] = px[q] - t1[a];
         for (int b = 0; b < np1; b++) {
-          const double d1 = py - t2[b];
-          const double r2ab = d0 * d0 + d1 * d1;
+          double d1[T], r2ab[T];
+#pragma unroll
+          for (int q = 0; q < T; q++) {
+            d1[q] = py[q] - t2[b];
+            r2ab[q] = d0[q] * d0[q] + d1[q] * d1[q];
+          }
 #pragma unroll 2
           for (int c = 0; c < np1; c++, kk++) {
             const double* m = M + (size_t)kk * NC;
-            const double d2 = pz - t3[c];
-            const double ri = rsqrt64(__builtin_fma(d2, d2, r2ab));
-            const double m0 = m[0] - sub * m[3], m1 = m[1] - sub * m[4], m2 = m[2] - sub * m[5];
-            const double tq = (m0 * d0 + m1 * d1 + m2 * d2) * (ri * ri);
-            u0 += (m0 + tq * d0) * ri;
-            u1 += (m1 + tq * d1) * ri;
-            u2 += (m2 + tq * d2) * ri;
+#pragma unroll
+            for (int q = 0; q < T; q++) {
+              const double d2 = pz[q] - t3[c];
+              const double ri = rsqrt64(__builtin_fma(d2, d2, r2ab[q]));
+              const double m0 = m[0] - sub[q] * m[3], m1 = m[1] - sub[q] * m[4], m2 = m[2] - sub[q] * m[5];
+              const double tq = (m0 * d0[q] + m1 * d1[q] + m2 * d2) * (ri * ri);
+              u0[q] += (m0 + tq * d0[q]) * ri;
+              u1[q] += (m1 + tq * d1[q]) * ri;
+              u2[q] += (m2 + tq * d2) * ri;
+            }
           }
         }
       }
     } else {
-      const double g0x = A.tsub2[i], g0y = A.tsub2[A.ntp + i], g0z = A.tsub2[2 * A.ntp + i];
+      double g0x[T], g0y[T], g0z[T];
+#pragma unroll
+      for (int q = 0; q < T; q++) {
+        g0x[q] = A.tsub2[ti[q]];
+        g0y[q] = A.tsub2[A.ntp + ti[q]];
+        g0z[q] = A.tsub2[2 * A.ntp + ti[q]];
+      }
       for (int a = 0, kk = 0; a < np1; a++) {
-        const double d0 = px - t1[a];
-        const double d00 = d0 * d0;
+        double d0[T], d00[T];
+#pragma unroll
+        for (int q = 0; q < T; q++) {
+          d0[q] = px[q] - t1[a];
+          d00[q] = d0[q] * d0[q];
+        }
         for (int b = 0; b < np1; b++) {
-          const double d1 = py - t2[b];
-          const double r2ab = __builtin_fma(d1, d1, d00);
-          const double gab = g0x * d0 + g0y * d1;
-          const double d11 = d1 * d1, d01 = d0 * d1;
-          double kab = 0.0;  // the row's kernel sum: d0 and d1 are fixed along it
+          double d1[T], r2ab[T], gab[T], d11[T], d01[T], kab[T];
+#pragma unroll
+          for (int q = 0; q < T; q++) {
+            d1[q] = py[q] - t2[b];
+            r2ab[q] = __builtin_fma(d1[q], d1[q], d00[q]);
+            gab[q] = g0x[q] * d0[q] + g0y[q] * d1[q];
+            d11[q] = d1[q] * d1[q];
+            d01[q] = d0[q] * d1[q];
+            kab[q] = 0.0;  // the row's kernel sum: d0 and d1 are fixed along it
+          }
 #pragma unroll 2
           for (int c = 0; c < np1; c++, kk++) {
             const double* m = M + (size_t)kk * NC;
-            const double d2 = pz - t3[c];
-            const double ri = rsqrt64(__builtin_fma(d2, d2, r2ab));
-            const double ri2 = ri * ri;
-            // d^T M d = M00 d0^2 + M11 d1^2 + S01 d0 d1 + d2 (S02 d0 + S12 d1 + M22 d2)
-            const double qab = __builtin_fma(m[3], d01, __builtin_fma(m[1], d11, m[0] * d00));
-            const double w = __builtin_fma(m[2], d2, __builtin_fma(m[5], d1, m[4] * d0));
-            const double wnd = m[6] * d0 + m[7] * d1 + m[8] * d2;
-            const double gd = __builtin_fma(g0z, d2, gab);
-            const double Q = __builtin_fma(-gd, wnd, __builtin_fma(w, d2, qab));
-            const double kern = Q * (ri2 * ri2 * ri);
-            kab += kern;
-            u2 += d2 * kern;
+#pragma unroll
+            for (int q = 0; q < T; q++) {
+              const double d2 = pz[q] - t3[c];
+              const double ri = rsqrt64(__builtin_fma(d2, d2, r2ab[q]));
+              const double ri2 = ri * ri;
+              // d^T M d = M00 d0^2 + M11 d1^2 + S01 d0 d1 + d2 (S02 d0 + S12 d1 + M22 d2)
+              const double qab = __builtin_fma(m[3], d01[q], __builtin_fma(m[1], d11[q], m[0] * d00[q]));
+              const double w = __builtin_fma(m[2], d2, __builtin_fma(m[5], d1[q], m[4] * d0[q]));
+              const double wnd = m[6] * d0[q] + m[7] * d1[q] + m[8] * d2;
+              const double gd = __builtin_fma(g0z[q], d2, gab[q]);
+              const double Q = __builtin_fma(-gd, wnd, __builtin_fma(w, d2, qab));
+              const double kern = Q * (ri2 * ri2 * ri);
+              kab[q] += kern;
+              u2[q] += d2 * kern;
+            }
           }
-          u0 += d0 * kab;
-          u1 += d1 * kab;
+#pragma unroll
+          for (int q = 0; q < T; q++) {
+            u0[q] += d0[q] * kab[q];
+            u1[q] += d1[q] * kab[q];
+          }
         }
       }
     }
   }
-  if (act) {
-    int s = A.slot[li];
-    A.slots[s] = u0;
-    A.slots[A.nslot + s] = u1;
-    A.slots[2 * A.nslot + s] = u2;
+#pragma unroll
+  for (int q = 0; q < T; q++) {
+    if (act[q]) {
+      const int s = A.slot[li[q]];
+      A.slots[s] = u0[q];
+      A.slots[A.nslot + s] = u1[q];
+      A.slots[2 * A.nslot + s] = u2[q];
+    }
   }
 }
+
+// Targets per lane of the far-field kernel: two in mixed precision (h = 1/256
+// mixed matvec 38.5 -> 38.0 ms), one in FP64, where two per lane halve the
+// occupancy (120 VGPRs) for the same time.
+constexpr int kFarT64 = 1, kFarT32 = 2;
 
 // Each target sums its slots in visiting order (one thread per target and
 // component), then v = scale*sum + xi*g0
@@ -841,13 +939,13 @@ int tree_plan_eval(hor_plan* P, const double* dens_dev, const double* d0_dev, do
     F.tgt = TP.far_tgt.p; F.slot = TP.far_slot.p; F.far_idx = MO.far_idx.p;
     F.cheb = TP.cheb.p; F.mom = MO.mom.p; F.tx = P->tx.p; F.tsub2 = P->tsub2.p; F.slots = TP.slots.p;
     F.node_ctr = TP.node_ctr.p; F.cheb32 = TP.cheb32.p; F.mom32 = MO.mom32.p;
-    dim3 grid(TP.n_far_items);
+    dim3 grid(TP.n_far_items), b64(kBlock / kFarT64), b32(kBlock / kFarT32);
     if (dl) {
-      if (mixed) hipLaunchKernelGGL((k_far<true, true>), grid, dim3(kBlock), 0, s, F);
-      else hipLaunchKernelGGL((k_far<true, false>), grid, dim3(kBlock), 0, s, F);
+      if (mixed) hipLaunchKernelGGL((k_far<true, true, kFarT32>), grid, b32, 0, s, F);
+      else hipLaunchKernelGGL((k_far<true, false, kFarT64>), grid, b64, 0, s, F);
     } else {
-      if (mixed) hipLaunchKernelGGL((k_far<false, true>), grid, dim3(kBlock), 0, s, F);
-      else hipLaunchKernelGGL((k_far<false, false>), grid, dim3(kBlock), 0, s, F);
+      if (mixed) hipLaunchKernelGGL((k_far<false, true, kFarT32>), grid, b32, 0, s, F);
+      else hipLaunchKernelGGL((k_far<false, false, kFarT64>), grid, b64, 0, s, F);
     }
     HOR_HIP_TRY(hipGetLastError());
   }

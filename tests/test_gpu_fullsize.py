@@ -97,3 +97,26 @@ def test_full_size_fused_matches_separate(ctx, band64):
         P.close()
         print("fused vs separate", kind, rel_max(half, ref), flush=True)
         assert rel_l2(half, ref) <= 1e-15 and rel_max(half, ref) <= 1e-15
+
+
+def test_full_size_mixed_precision(ctx, oracle, band64):
+    """The mixed-precision fused pass over all 1,314,272 nodes: within 1e-6 of
+    the FP64 pass (2-norm), and its error against the oracle on the seeded
+    1,500-node sample stated."""
+    body, s, t, DEL = band64
+    n = t["n"]
+    fg, fg0 = np.concatenate([s["f"], s["g"]]), np.concatenate([t["f0"], t["g0"]])
+    P64 = hor.Plan(ctx, hor.SLDL_OFF, s, t, DEL)
+    out64 = P64(fg, fg0)
+    P64.close()
+    P32 = hor.Plan(ctx, hor.SLDL_OFF, s, t, DEL, precision=hor.MIXED)
+    out32 = P32(fg, fg0)
+    P32.close()
+    idx = np.sort(np.random.default_rng(64).choice(n, 1500, replace=False))
+    sub = _sample(t, idx)
+    for name, half, orc in (("SL", slice(0, 3 * n), oracle.sl_off(DEL, s, sub)),
+                            ("DL", slice(3 * n, 6 * n), oracle.dl_off(DEL, s, sub))):
+        e64 = rel_l2(out32[half], out64[half])
+        eo = rel_l2(out32[half].reshape(-1, 3)[idx].ravel(), orc)
+        print(f"mixed {name}: vs FP64 over all nodes {e64:.2e}, vs the oracle on 1500 nodes {eo:.2e}", flush=True)
+        assert e64 <= 1e-6 and eo <= 1e-6
