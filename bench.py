@@ -293,6 +293,7 @@ def gmres_workload(h=1.0 / 128.0, gap=0.1, P=8, N0=4000, theta=0.6, tol=1e-8, se
                                    "profiles/tree_matvec_h256_pmc.json")
         ctx.free(ud)
         ctx.free(yd)
+        out["sl_far_field"] = sl_far_kernels(ctx, pd, DEL, tp)
     st = S64.stats()
     mv_s = out["matvec_ms"]["fp64"] * 1e-3
     out["matvec_work"] = dict(st, direct_equivalent_pairs_per_s=st["direct_pairs"] / mv_s,
@@ -445,6 +446,41 @@ def phase_mean(S, u_dev, y_dev, reps):
     return acc
 
 
+def sl_far_kernels(ctx, pd, DEL, tp, reps=3):
+    """Single-layer far field of the forcing right-hand side -2/3 SL[f]
+    (main.cpp:199-207): sphere 1's traction at its own points (SL_ON) and at
+    sphere 2's (SL_OFF), mixed precision, the proxy interactions on the FP32
+    vector units (k_far) and as a GEMM on FP32 matrix cores (k_far_sl_mfma).
+    Far-phase kernel time (HIP events on the plan's stream, mean of reps
+    evaluations after one warm-up) and the relative l2 difference of each
+    result from the FP64 plan's."""
+    import hor
+    s = pd[0]
+    out = {}
+    for name, kind, tgt, d0 in (("on_surface", hor.SL_ON, None, None),
+                                ("other_sphere", hor.SL_OFF, pd[1], pd[1]["f0"])):
+        ref_plan = hor.Plan(ctx, kind, s, tgt, DEL, hor.TREE, hor.FP64, s["ctr"], tp)
+        ref = ref_plan(s["f"], d0)
+        ref_plan.close()
+        row = {}
+        for kname, kern in (("valu", hor.FAR_VALU), ("mfma", hor.FAR_MFMA)):
+            plan = hor.Plan(ctx, kind, s, tgt, DEL, hor.TREE, hor.MIXED, s["ctr"], tp).set_far_kernel(kern)
+            y = plan(s["f"], d0)
+            far = 0.0
+            for _ in range(reps):
+                y = plan(s["f"], d0)
+                far += plan.phase_timing()["far_ms"]
+            row[kname] = {"far_ms": far / reps,
+                          "rel_l2_vs_fp64": float(np.linalg.norm(y - ref) / np.linalg.norm(ref))}
+            row["proxy_interactions"] = plan.stats()["proxy_pairs"]
+            plan.close()
+        row["mfma_time_over_valu"] = row["mfma"]["far_ms"] / row["valu"]["far_ms"]
+        out[name] = row
+        log(f"SL far field {name}: FP32 VALU {row['valu']['far_ms']:.2f} ms, "
+            f"FP32 MFMA {row['mfma']['far_ms']:.2f} ms")
+    return out
+
+
 def matvec_workload(h=1.0 / 256.0, gap=0.1, P=10, N0=8000, theta=0.6, reps=3, seed=SEED):
     """Only the two-sphere treecode matvecs (the GMRES operator): reps FP64
     and reps mixed-precision matvecs of the seeded rigid-body vector, after one
@@ -472,6 +508,7 @@ def matvec_workload(h=1.0 / 256.0, gap=0.1, P=10, N0=8000, theta=0.6, reps=3, se
         ctx.free(ud)
         ctx.free(yd)
         S.close()
+    out["sl_far_field"] = sl_far_kernels(ctx, pd, DEL, tp)
     ctx.close()
     return out
 

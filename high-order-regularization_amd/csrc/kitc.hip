@@ -68,12 +68,17 @@ struct TreePlan {
   // moments (its far nodes included) and is evaluated first on the same
   // density: the far field reads that plan's far_idx, mom and mom32.
   const TreePlan* mom_owner = nullptr;
+  // mixed-precision single-layer far field: HOR_FAR_VALU (k_far) or
+  // HOR_FAR_MFMA (k_far_sl_mfma over the per-proxy columns in bmat)
+  int far_kernel = HOR_FAR_VALU;
+  DBuf<float> bmat;
   // reduction
   DBuf<int> slot_start;      // sorted target order, nt+1
   DBuf<double> slots;        // 3*nslot
   // phase events of the last evaluation: before the moments, after the
   // moments, after the near sums, after the far field
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  bool evaluated = false;
   ~TreePlan() {
     for (hipEvent_t& e : ev)
       if (e) (void)hipEventDestroy(e);
@@ -534,6 +539,175 @@ __global__ __launch_bounds__(kBlock / T) void k_far(FarArgs A) {
 // occupancy (120 VGPRs) for the same time.
 constexpr int kFarT64 = 1, kFarT32 = 2;
 
+// ---------------------------------------------------------------------------
+// Single-layer far field as a GEMM on FP32 matrix cores (mixed precision,
+// chosen per plan with hor_plan_set_far_kernel).  The proxy interaction of
+// Call_Treecode_SL (KITC.cpp:671-720), w/r + d (w.d)/r^3 with d = p - y in
+// node-centred coordinates (p the target, y the proxy), is K3 = 1/r^3 times
+// r^2 w + d (w.d), a polynomial in p and y.  Per target the sums over the
+// proxies of K3 times
+//   S = w, T = w.y, U_ij = y_i w_j, V = y (w.y), Z = |y|^2 w
+// (19 columns per moment set) give
+//   u = |p|^2 S - 2 U^T p + Z + p (p.S) - p T - U p + V,
+// and the subtraction's two moment sets (w = W_f, W_n) make 38 columns of the
+// B operand, padded to 48, whose rows are the node's proxies.  The A operand
+// [targets x proxies] is K3 from the VALU.  Each v_mfma_f32_16x16x4f32 takes
+// 16 targets x 4 proxies x 16 columns; the FP32 products are summed over
+// kFarMfmaFlush groups of four proxies on the matrix core and then added in
+// FP64, and the per-target combination runs in FP64.
+constexpr int kFarMfmaCols = 48;   // 2 x 19 moment columns, 2 zero, proxy position at 40..42, 5 zero
+constexpr int kFarMfmaFlush = 8;   // 32 proxies per FP32 partial sum (k_far sums rows of P + 1)
+
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+// B rows of every far node of the moment owner: [far node][proxy][48], the
+// proxies padded to a multiple of four with zero moment columns (their
+// position is the last proxy's, so K3 stays finite).
+__global__ void k_far_sl_bmat(int nfar, int np1, int pflat, int pflat4, const int* far_node, const float* cheb32,
+                              const float* mom32, float* B) {
+  const int fi = blockIdx.y;
+  const int kk = blockIdx.x * blockDim.x + threadIdx.x;
+  if (fi >= nfar || kk >= pflat4) return;
+  const int kc = min(kk, pflat - 1);
+  const int a = kc / (np1 * np1), b = (kc / np1) % np1, c = kc % np1;
+  const float* t = cheb32 + (size_t)3 * np1 * far_node[fi];
+  const double y[3] = {t[a], t[np1 + b], t[2 * np1 + c]};
+  const double yy = y[0] * y[0] + y[1] * y[1] + y[2] * y[2];
+  const float* m = mom32 + ((size_t)fi * pflat + kc) * kNc32Sl;
+  float* o = B + ((size_t)fi * pflat4 + kk) * kFarMfmaCols;
+  const bool live = kk < pflat;
+  for (int s = 0; s < 2; s++) {
+    double w[3];
+    for (int j = 0; j < 3; j++) w[j] = live ? (double)m[3 * s + j] : 0.0;
+    const double wy = w[0] * y[0] + w[1] * y[1] + w[2] * y[2];
+    float* col = o + 19 * s;
+    for (int j = 0; j < 3; j++) {
+      col[j] = (float)w[j];
+      col[13 + j] = (float)(y[j] * wy);
+      col[16 + j] = (float)(yy * w[j]);
+      for (int i = 0; i < 3; i++) col[4 + 3 * i + j] = (float)(y[i] * w[j]);
+    }
+    col[3] = (float)wy;
+  }
+  o[38] = o[39] = 0.0f;
+  for (int j = 0; j < 3; j++) o[40 + j] = (float)y[j];
+  for (int j = 43; j < kFarMfmaCols; j++) o[j] = 0.0f;
+}
+
+struct FarMfmaArgs {
+  int pflat4, ntp, nslot;
+  const int *item_node, *item_off, *item_cnt, *tgt, *slot, *far_idx;
+  const double *tx, *tsub2, *node_ctr;
+  const float* B;
+  double* slots;
+};
+
+// u of one moment set from its 19 column sums (the formula above), FP64.
+__device__ __forceinline__ void sl_far_combine(const double* v, const double p[3], double u[3]) {
+  const double pp = p[0] * p[0] + p[1] * p[1] + p[2] * p[2];
+  const double pS = p[0] * v[0] + p[1] * v[1] + p[2] * v[2];
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    double utp = 0.0, up = 0.0;   // (U^T p)_i and (U p)_i
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      utp += p[k] * v[4 + 3 * k + i];
+      up += v[4 + 3 * i + k] * p[k];
+    }
+    u[i] = pp * v[i] - 2.0 * utp + v[16 + i] + p[i] * (pS - v[3]) - up + v[13 + i];
+  }
+}
+
+// One work item is a far node and up to 256 of its targets (as k_far); wave
+// w takes targets [64 w, 64 w + 64) as four tiles of 16 rows.  Lane l
+// supplies the A value of target row l % 16 and proxy l / 16 of each group of
+// four, and the B values of that proxy at column l % 16 of the three column
+// tiles; result row 4 (l / 16) + r and column l % 16 of a tile are in
+// element r of the lane's accumulator.
+__global__ __launch_bounds__(kBlock) void k_far_sl_mfma(FarMfmaArgs A) {
+  __shared__ double tile[kBlock / kWave][16][kFarMfmaCols + 1];
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  const int item = blockIdx.x;
+  const int node = A.item_node[item], off = A.item_off[item], cnt = A.item_cnt[item];
+  const int w0 = wave * kWave;
+  if (w0 >= cnt) return;   // a wave without targets (no workgroup barrier here)
+  const double cx = A.node_ctr[3 * node], cy = A.node_ctr[3 * node + 1], cz = A.node_ctr[3 * node + 2];
+  const int l16 = lane & 15, kq = lane >> 4;
+  float qx[4], qy[4], qz[4];
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const int k = w0 + 16 * r + l16;
+    const int ti = A.tgt[off + (k < cnt ? k : 0)];   // rows past the end repeat the item's first target
+    qx[r] = (float)(A.tx[ti] - cx);
+    qy[r] = (float)(A.tx[A.ntp + ti] - cy);
+    qz[r] = (float)(A.tx[2 * A.ntp + ti] - cz);
+  }
+  const float* Bn = A.B + (size_t)A.far_idx[node] * A.pflat4 * kFarMfmaCols;
+  f32x4_t acc[4][3];
+  double s64[4][3][4];
+#pragma unroll
+  for (int r = 0; r < 4; r++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      acc[r][c] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int i = 0; i < 4; i++) s64[r][c][i] = 0.0;
+    }
+  const int ng = A.pflat4 / 4;
+  for (int g0 = 0; g0 < ng; g0 += kFarMfmaFlush) {
+    const int g1 = min(ng, g0 + kFarMfmaFlush);
+    for (int g = g0; g < g1; g++) {
+      const float* row = Bn + (size_t)(4 * g + kq) * kFarMfmaCols;
+      const float b0 = row[l16], b1 = row[16 + l16], b2 = row[32 + l16];
+      const float yx = row[40], yy = row[41], yz = row[42];
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const float dx = qx[r] - yx, dy = qy[r] - yy, dz = qz[r] - yz;
+        const float ri = rsqrt32(__builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)));
+        const float k3 = ri * ri * ri;
+        acc[r][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(k3, b0, acc[r][0], 0, 0, 0);
+        acc[r][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(k3, b1, acc[r][1], 0, 0, 0);
+        acc[r][2] = __builtin_amdgcn_mfma_f32_16x16x4f32(k3, b2, acc[r][2], 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) s64[r][c][i] += (double)acc[r][c][i];
+        acc[r][c] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+      }
+  }
+  // per tile: column sums through this wave's LDS rows, then one lane per
+  // target row combines them (LDS operations of one wave complete in order)
+  double(*T)[kFarMfmaCols + 1] = tile[wave];
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    if (w0 + 16 * r >= cnt) break;   // wave-uniform
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+      for (int i = 0; i < 4; i++) T[4 * kq + i][16 * c + l16] = s64[r][c][i];
+    __builtin_amdgcn_wave_barrier();
+    const int k = w0 + 16 * r + l16;
+    if (kq == 0 && k < cnt) {
+      const int li = off + k;
+      const int ti = A.tgt[li];
+      const double p[3] = {A.tx[ti] - cx, A.tx[A.ntp + ti] - cy, A.tx[2 * A.ntp + ti] - cz};
+      double uf[3], un[3];
+      sl_far_combine(T[l16], p, uf);
+      sl_far_combine(T[l16] + 19, p, un);
+      const double sub = A.tsub2[ti];
+      const int s = A.slot[li];
+      A.slots[s] = uf[0] - sub * un[0];
+      A.slots[A.nslot + s] = uf[1] - sub * un[1];
+      A.slots[2 * A.nslot + s] = uf[2] - sub * un[2];
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
 // Each target sums its slots in visiting order (one thread per target and
 // component), then v = scale*sum + xi*g0
 // combined into out as the direct finalize does.
@@ -940,7 +1114,21 @@ int tree_plan_eval(hor_plan* P, const double* dens_dev, const double* d0_dev, do
     F.cheb = TP.cheb.p; F.mom = MO.mom.p; F.tx = P->tx.p; F.tsub2 = P->tsub2.p; F.slots = TP.slots.p;
     F.node_ctr = TP.node_ctr.p; F.cheb32 = TP.cheb32.p; F.mom32 = MO.mom32.p;
     dim3 grid(TP.n_far_items), b64(kBlock / kFarT64), b32(kBlock / kFarT32);
-    if (dl) {
+    if (mixed && !dl && TP.far_kernel == HOR_FAR_MFMA) {
+      const int pflat4 = (TP.pflat + 3) & ~3;
+      const size_t need = (size_t)std::max(MO.nfar_nodes, 1) * pflat4 * kFarMfmaCols;
+      if (TP.bmat.n < need) HOR_TRY(TP.bmat.alloc(need));
+      if (MO.nfar_nodes > 0)
+        hipLaunchKernelGGL(k_far_sl_bmat, dim3((pflat4 + kBlock - 1) / kBlock, MO.nfar_nodes), dim3(kBlock), 0, s,
+                           MO.nfar_nodes, TP.np1, TP.pflat, pflat4, MO.far_node.p, TP.cheb32.p, MO.mom32.p,
+                           TP.bmat.p);
+      FarMfmaArgs G;
+      G.pflat4 = pflat4; G.ntp = P->ntp; G.nslot = TP.nslot;
+      G.item_node = F.item_node; G.item_off = F.item_off; G.item_cnt = F.item_cnt;
+      G.tgt = F.tgt; G.slot = F.slot; G.far_idx = F.far_idx;
+      G.tx = F.tx; G.tsub2 = F.tsub2; G.node_ctr = F.node_ctr; G.B = TP.bmat.p; G.slots = F.slots;
+      hipLaunchKernelGGL(k_far_sl_mfma, grid, dim3(kBlock), 0, s, G);
+    } else if (dl) {
       if (mixed) hipLaunchKernelGGL((k_far<true, true, kFarT32>), grid, b32, 0, s, F);
       else hipLaunchKernelGGL((k_far<true, false, kFarT64>), grid, b64, 0, s, F);
     } else {
@@ -951,6 +1139,7 @@ int tree_plan_eval(hor_plan* P, const double* dens_dev, const double* d0_dev, do
   }
   HOR_HIP_TRY(hipEventRecord(c->ev[3], s));
   HOR_HIP_TRY(hipEventRecord(TP.ev[3], s));
+  TP.evaluated = true;
   // 4. per-target reduction
   hipLaunchKernelGGL(k_tree_finalize, dim3((P->nt + kBlock - 1) / kBlock, 3), dim3(kBlock), 0, s, P->nt, P->ntp,
                      TP.nslot, P->tperm.p, TP.slot_start.p, TP.slots.p,
@@ -960,12 +1149,23 @@ int tree_plan_eval(hor_plan* P, const double* dens_dev, const double* d0_dev, do
   return HOR_OK;
 }
 
+int tree_plan_set_far_kernel(hor_plan* P, int kernel) {
+  if (!P || !P->tree) return set_error(HOR_E_ARG, "far-field kernel: not a treecode plan");
+  if (kernel != HOR_FAR_VALU && kernel != HOR_FAR_MFMA) return set_error(HOR_E_ARG, "far-field kernel: bad value");
+  if (kernel == HOR_FAR_MFMA &&
+      (P->precision != HOR_MIXED || (P->kind != HOR_SL_ON && P->kind != HOR_SL_OFF)))
+    return set_error(HOR_E_ARG, "the matrix-core far field is for mixed-precision single-layer treecode plans");
+  P->tree->far_kernel = kernel;
+  return HOR_OK;
+}
+
 // Kernel time of each phase of the plan's last evaluation (HIP events on the
 // context's stream; call after the stream has synchronized): moments (moment
 // right-hand side, GEMM and reduction), near sums (k_leaf), far field (k_far).
 int tree_plan_phase_ms(hor_plan* P, double ms[3]) {
-  if (!P->tree) return set_error(HOR_E_STATE, "not a treecode plan");
+  if (!P->tree) return set_error(HOR_E_ARG, "not a treecode plan");
   const TreePlan& TP = *P->tree;
+  if (!TP.evaluated) return set_error(HOR_E_STATE, "phase timing before the plan's first evaluation");
   for (int k = 0; k < 3; k++) {
     float t = 0;
     HOR_HIP_TRY(hipEventElapsedTime(&t, TP.ev[k], TP.ev[k + 1]));

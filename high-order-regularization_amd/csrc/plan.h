@@ -68,6 +68,8 @@ int tree_plan_create(hor_plan* P, const double* sx, const double* sn, const doub
 int tree_plan_share_moments(hor_plan* owner, hor_plan* user, const double* sx);
 int tree_plan_eval(hor_plan* P, const double* dens_dev, const double* d0_dev, double* out_dev,
                    const double* base, double coef, int accumulate);
+// HOR_FAR_VALU or HOR_FAR_MFMA (mixed-precision single-layer plans only).
+int tree_plan_set_far_kernel(hor_plan* P, int kernel);
 // Kernel time of each phase of a treecode plan's last evaluation: moments,
 // near sums, far field (ms).
 int tree_plan_phase_ms(hor_plan* P, double ms[3]);

@@ -505,6 +505,20 @@ extern "C" int hor_plan_stats(hor_plan* P, double* direct_pairs, double* near_pa
   return HOR_OK;
 }
 
+extern "C" int hor_plan_set_far_kernel(hor_plan* P, int kernel) { return hor::tree_plan_set_far_kernel(P, kernel); }
+
+extern "C" int hor_plan_phase_timing(hor_plan* P, double* moments_ms, double* near_ms, double* far_ms) {
+  if (!P) return set_error(HOR_E_ARG, "null plan");
+  double ms[3];
+  HOR_HIP_TRY(hipSetDevice(P->ctx->device));
+  HOR_HIP_TRY(hipStreamSynchronize(P->ctx->stream));
+  HOR_TRY(hor::tree_plan_phase_ms(P, ms));
+  if (moments_ms) *moments_ms = ms[0];
+  if (near_ms) *near_ms = ms[1];
+  if (far_ms) *far_ms = ms[2];
+  return HOR_OK;
+}
+
 // ---------------------------------------------------------------------------
 // One-shot drop-ins
 

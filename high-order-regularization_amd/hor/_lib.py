@@ -18,6 +18,7 @@ SL_ON, SL_OFF, DL_ON, DL_OFF, P_OFF, SLDL_OFF = 0, 1, 2, 3, 4, 5
 ALL_NEAR, ON_AT = 0x10, 0x20   # kind modifier bits (hor.h)
 DIRECT, TREE = 0, 1
 FP64, MIXED = 0, 1
+FAR_VALU, FAR_MFMA = 0, 1   # far-field kernel of a mixed single-layer treecode plan
 
 
 class TreeParams(C.Structure):
@@ -57,6 +58,8 @@ _SIGS = {
     "hor_plan_eval": (_I, [_P, _P, _P, _P]),
     "hor_plan_eval_dev": (_I, [_P, _P, _P, _P]),
     "hor_plan_stats": (_I, [_P, _DP, _DP, _DP, _DP]),
+    "hor_plan_set_far_kernel": (_I, [_P, _I]),
+    "hor_plan_phase_timing": (_I, [_P, _DP, _DP, _DP]),
     "hor_sl_offsurf": (_I, [_P, _D, _I, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P]),
     "hor_sl_onsurf": (_I, [_P, _D, _I, _P, _P, _P, _P, _I, _P, _P]),
     "hor_dl_offsurf": (_I, [_P, _D, _I, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P]),

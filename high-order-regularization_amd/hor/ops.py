@@ -14,7 +14,8 @@ import weakref
 import numpy as np
 
 from . import _lib
-from ._lib import DIRECT, DL_OFF, DL_ON, FP64, MIXED, P_OFF, SL_OFF, SLDL_OFF, SL_ON, TREE, check, f64, ptr
+from ._lib import (DIRECT, DL_OFF, DL_ON, FAR_MFMA, FAR_VALU, FP64, MIXED, P_OFF, SL_OFF, SLDL_OFF, SL_ON, TREE,  # noqa: F401
+                   check, f64, ptr)
 
 __all__ = ["Context", "Plan", "TreeParams", "SL_ON", "SL_OFF", "DL_ON", "DL_OFF", "P_OFF", "SLDL_OFF", "DIRECT", "TREE",
            "FP64", "MIXED", "eval_Stokes_SL_offSurf", "eval_Stokes_SL_onSurf",
@@ -146,6 +147,19 @@ class Plan:
 
     def eval_dev(self, dens_dev, d0_dev, out_dev):
         check(_lib.lib().hor_plan_eval_dev(self.h, dens_dev, d0_dev, out_dev), "hor_plan_eval_dev")
+
+    def set_far_kernel(self, kernel):
+        """FAR_VALU or FAR_MFMA (hor_plan_set_far_kernel): the far-field kernel
+        of a mixed-precision single-layer treecode plan."""
+        check(_lib.lib().hor_plan_set_far_kernel(self.h, int(kernel)), "hor_plan_set_far_kernel")
+        return self
+
+    def phase_timing(self):
+        """Kernel ms of the last evaluation's moments, near sums and far field
+        (treecode plans; hor_plan_phase_timing)."""
+        v = [C.c_double() for _ in range(3)]
+        check(_lib.lib().hor_plan_phase_timing(self.h, *[C.byref(x) for x in v]), "hor_plan_phase_timing")
+        return dict(moments_ms=v[0].value, near_ms=v[1].value, far_ms=v[2].value)
 
     def stats(self):
         v = [C.c_double() for _ in range(4)]

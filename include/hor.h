@@ -147,6 +147,22 @@ int hor_plan_eval_dev(hor_plan* plan, const double* dens_dev, const double* d0_d
 int hor_plan_stats(hor_plan* plan, double* direct_pairs, double* near_pairs,
                    double* proxy_pairs, double* moment_terms);
 
+/* Far-field kernel of a mixed-precision single-layer treecode plan (the proxy
+ * interactions of Call_Treecode_SL, KITC.cpp:671-720):
+ *   HOR_FAR_VALU (default): FP32 vector arithmetic per proxy interaction;
+ *   HOR_FAR_MFMA: the interactions as a GEMM on FP32 matrix cores,
+ *     [targets x proxies] 1/r^3 times [proxies x 38] moment polynomials in
+ *     node-centred coordinates, FP64 combination per target.
+ * Any other plan accepts only HOR_FAR_VALU (HOR_E_ARG otherwise). */
+enum { HOR_FAR_VALU = 0, HOR_FAR_MFMA = 1 };
+int hor_plan_set_far_kernel(hor_plan* plan, int kernel);
+
+/* Kernel time of each phase of a treecode plan's last evaluation, in ms (HIP
+ * events on the context's stream): moments, near sums, far field (the
+ * matrix-core far field includes building its B operand).  HOR_E_STATE
+ * before the first evaluation; HOR_E_ARG for a direct plan. */
+int hor_plan_phase_timing(hor_plan* plan, double* moments_ms, double* near_ms, double* far_ms);
+
 /* ------------------------------------------------------------------------
  * One-shot drop-ins of the reference functions (plan, eval, destroy).
  * ------------------------------------------------------------------------ */

@@ -76,6 +76,40 @@ def test_tree_mixed_vs_fp64(ctx, ell):
         assert rel_l2(a, b) <= 1e-6
 
 
+@pytest.mark.parametrize("P,N0", [(8, 300), (10, 400)])
+def test_tree_sl_far_mfma(ctx, oracle, ell, P, N0):
+    """The mixed single-layer far field on FP32 matrix cores (FAR_MFMA) against
+    the oracle's FP64 treecode, on-surface and off-surface: within the mixed
+    path's 1e-6, repeatable, and close to the FP32 vector kernel (FAR_VALU).
+    (P + 1)^3 = 729 and 1331 proxies are not multiples of four, so the padded
+    proxy rows are exercised.  Other plans refuse the matrix-core far field."""
+    body, s, t, DEL = ell
+    tp_o = oracle.tree_params(P, N0, 0.6, body.ctr)
+    tp = hor.TreeParams(P, N0, 0.6)
+    cases = ((hor.SL_ON, None, None, oracle.sl_on_tree(tp_o, DEL, s)),
+             (hor.SL_OFF, t, t["f0"], oracle.sl_off_tree(tp_o, DEL, s, t)))
+    for kind, tgt, d0, ref in cases:
+        valu = hor.Plan(ctx, kind, s, tgt, DEL, hor.TREE, hor.MIXED, body.ctr, tp)(s["f"], d0)
+        plan = hor.Plan(ctx, kind, s, tgt, DEL, hor.TREE, hor.MIXED, body.ctr, tp).set_far_kernel(hor.FAR_MFMA)
+        assert plan.stats()["proxy_pairs"] > 0
+        mfma = plan(s["f"], d0)
+        again = plan(s["f"], d0)
+        e_m, e_v, d_mv = rel_l2(mfma, ref), rel_l2(valu, ref), rel_l2(mfma, valu)
+        print(f"SL kind {kind} P={P}: FAR_MFMA vs oracle {e_m:.2e} (max {rel_max(mfma, ref):.2e}), "
+              f"FAR_VALU vs oracle {e_v:.2e}, MFMA vs VALU {d_mv:.2e}", flush=True)
+        assert np.all(np.isfinite(mfma))
+        assert np.array_equal(mfma, again)
+        assert e_m <= 1e-6
+        assert d_mv <= 1e-6
+    plan = hor.Plan(ctx, hor.DL_ON, s, None, DEL, hor.TREE, hor.MIXED, body.ctr, tp)
+    with pytest.raises(hor.HorError):
+        plan.set_far_kernel(hor.FAR_MFMA)
+    plan = hor.Plan(ctx, hor.SL_ON, s, None, DEL, hor.TREE, hor.FP64, body.ctr, tp)
+    with pytest.raises(hor.HorError):
+        plan.set_far_kernel(hor.FAR_MFMA)
+    plan.set_far_kernel(hor.FAR_VALU)
+
+
 def test_tree_converges_in_P(ctx, ell):
     """The treecode converges as P grows (self-convergence against P = 10).
     Its limit is not the regularized direct sum: the far field always uses the
