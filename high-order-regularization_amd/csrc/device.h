@@ -140,6 +140,10 @@ __device__ __forceinline__ float rsqrt32(float x) { return __builtin_amdgcn_rsqf
 constexpr int kDlOnTabLen = kDlOnIntervals * (kDlOnDeg + 1);
 constexpr int kAETabLen = kAEIntervals * 2 * (kAEDeg + 1);
 __host__ __device__ inline int table_len(int kind) { return kind == HOR_DL_ON ? kDlOnTabLen : kAETabLen; }
+// In LDS the tables are coefficient-major: coefficient j of every interval is
+// contiguous, so the lanes of a wave, whose arguments fall in neighbouring
+// intervals, read neighbouring words (distinct LDS banks, or one broadcast).
+constexpr int kAERow = 2 * (kAEDeg + 1);   // doubles per interval in erf_exp_table.h (A, then E)
 
 // Source record read by the sum kernels, one per sorted source: position,
 // weight a, a-scaled normal a n, and the density, a-scaled for the single layer

@@ -16,7 +16,12 @@ template <int KIND>
 __device__ __forceinline__ const double* stage_table(const DirectArgs& A) {
   extern __shared__ double lds_tab[];
   const double* src = KIND == HOR_DL_ON ? A.dlon_tab : A.ae_tab;
-  for (int q = threadIdx.x; q < table_len(KIND); q += blockDim.x) lds_tab[q] = src[q];
+  if (KIND == HOR_DL_ON) {
+    for (int q = threadIdx.x; q < kDlOnTabLen; q += blockDim.x)
+      lds_tab[(q % (kDlOnDeg + 1)) * kDlOnIntervals + q / (kDlOnDeg + 1)] = src[q];
+  } else {
+    for (int q = threadIdx.x; q < kAETabLen; q += blockDim.x) lds_tab[(q % kAERow) * kAEIntervals + q / kAERow] = src[q];
+  }
   __syncthreads();
   return lds_tab;
 }
@@ -26,12 +31,13 @@ __device__ __forceinline__ const double* stage_table(const DirectArgs& A) {
 __device__ __forceinline__ void erf_exp_AE(const double* __restrict__ tab, double s, double& A, double& E) {
   const int k = min((int)(s * (kAEIntervals / kAESmax)), kAEIntervals - 1);
   const double x = s * (2.0 * kAEIntervals / kAESmax) - (double)(2 * k + 1);
-  const double* c = tab + k * 2 * (kAEDeg + 1);
-  double a = c[kAEDeg], e = c[2 * kAEDeg + 1];
+  const double* c = tab + k;
+  constexpr int S = kAEIntervals;
+  double a = c[kAEDeg * S], e = c[(2 * kAEDeg + 1) * S];
 #pragma unroll
   for (int j = kAEDeg - 1; j >= 0; j--) {
-    a = __builtin_fma(a, x, c[j]);
-    e = __builtin_fma(e, x, c[kAEDeg + 1 + j]);
+    a = __builtin_fma(a, x, c[j * S]);
+    e = __builtin_fma(e, x, c[(kAEDeg + 1 + j) * S]);
   }
   A = a;
   E = e;
@@ -42,10 +48,11 @@ __device__ __forceinline__ void erf_exp_AE(const double* __restrict__ tab, doubl
 __device__ __forceinline__ double dl_on_G(const double* __restrict__ tab, double s) {
   const int k = min((int)(s * (kDlOnIntervals / kDlOnSmax)), kDlOnIntervals - 1);
   const double x = s * (2.0 * kDlOnIntervals / kDlOnSmax) - (double)(2 * k + 1);
-  const double* c = tab + k * (kDlOnDeg + 1);
-  double g = c[kDlOnDeg];
+  const double* c = tab + k;
+  constexpr int S = kDlOnIntervals;
+  double g = c[kDlOnDeg * S];
 #pragma unroll
-  for (int j = kDlOnDeg - 1; j >= 0; j--) g = __builtin_fma(g, x, c[j]);
+  for (int j = kDlOnDeg - 1; j >= 0; j--) g = __builtin_fma(g, x, c[j * S]);
   return g;
 }
 
