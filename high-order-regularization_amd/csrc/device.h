@@ -89,8 +89,11 @@ struct DBuf {
 //       (utilities.cpp:310-315).
 constexpr int kNumCoef = 12;
 
-// Fills the 12 polynomial coefficients from c1, c2, c3.
-inline void poly_coefs(const double c[3], double q[kNumCoef]) {
+// Fills the 12 polynomial coefficients from c1, c2, c3 (a target stores c,
+// the sum kernels expand it when they load the target).  No contraction into
+// fused multiply-adds: host and device round alike.
+__host__ __device__ inline void poly_coefs(const double c[3], double q[kNumCoef]) {
+#pragma clang fp contract(off)
   const double rp = 1.7724538509055160273;  // sqrt(pi)
   double c1 = c[0], c2 = c[1], c3 = c[2];
   q[0] = c1;
@@ -107,16 +110,6 @@ inline void poly_coefs(const double c[3], double q[kNumCoef]) {
   q[11] = 16.0 / 3.0 * c3;
 }
 
-// On-surface stresslet polynomial (7th order), already multiplied by 2/sqrt(pi).
-inline void poly_coefs_dl_on(double q[kNumCoef]) {
-  const double k = 2.0 / 1.7724538509055160273;
-  for (int i = 0; i < kNumCoef; i++) q[i] = 0.0;
-  q[7] = -1.0 * k;
-  q[8] = -2.0 / 3.0 * k;
-  q[9] = 52.0 / 45.0 * k;
-  q[10] = -8.0 / 45.0 * k;
-  q[11] = 0.0;
-}
 
 // ---------------------------------------------------------------------------
 // Device math.

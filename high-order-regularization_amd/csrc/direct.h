@@ -29,7 +29,7 @@ struct DirectArgs {
                                    // y - (centre of the source's 64-subtile box)
   const double* sub_box;           // 6 per 64-source subtile (lo xyz, hi xyz)
   const double* tx;                // 3*ntp positions
-  const double* tcoef;             // 12*ntp regularization polynomial coefficients
+  const double* tcoef;             // 3*ntp regularization coefficients c1, c2, c3 (zero beyond 8 delta)
   const double* tsub;              // SL: ntp f0.n0; DL: 3*ntp g0; P: ntp f0.n0, 3*ntp n0 x f0;
                                    // SLDL: ntp f0.n0, 3*ntp g0
   const double* tx0;               // DL_OFF: 3*ntp nearest point

@@ -87,7 +87,7 @@ Layout layout(int ns, int nt) {
   L.sub_box = take(d * 6 * std::max(L.nsub, 1));
   L.sperm = take(sizeof(int) * L.nsp);
   L.tx = take(d * 3 * L.ntp);
-  L.tcoef = take(d * kNumCoef * L.ntp);
+  L.tcoef = take(d * 3 * L.ntp);
   L.tnear = take(sizeof(int) * L.ntp);
   L.wave_box = take(d * 8 * (L.ntp / kWave));
   L.xi = take(d * L.ntp);
@@ -99,7 +99,7 @@ Layout layout(int ns, int nt) {
 }
 
 struct Coefs {
-  double q[kNumCoef];
+  double c[3];   // regularization coefficients c1, c2, c3 (target_coefs)
 };
 
 // Sources in the caller's order as SoA with stride nsp (zero padding).
@@ -117,14 +117,15 @@ __global__ void k_na_sources(int ns, int nsp, const double* sx, const double* sn
 }
 
 // Targets in the caller's order, padded to ntp with copies of the last one:
-// positions (SoA), the on-surface polynomial, class "near", jump 1/2.
+// positions (SoA), zero regularization coefficients (the on-surface stresslet
+// takes its kernel from its table), class "near", jump 1/2.
 __global__ void k_na_targets(int nt, int ntp, const double* tx, Coefs c, double* X, double* C, int* nearp,
                              double* xi, int* tperm) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ntp) return;
   const int o = min(i, nt - 1);
   for (int k = 0; k < 3; k++) X[(size_t)k * ntp + i] = tx[3 * (size_t)o + k];
-  for (int k = 0; k < kNumCoef; k++) C[(size_t)k * ntp + i] = c.q[k];
+  for (int k = 0; k < 3; k++) C[(size_t)k * ntp + i] = c.c[k];
   nearp[i] = 1;
   xi[i] = i < nt ? 0.5 : 0.0;   // stresslets.cpp:276-278
   tperm[i] = o;
@@ -186,7 +187,7 @@ extern "C" int hor_dl_onsurf_dev_noalloc(hor_ctx* ctx, double DEL, int ns, const
                      D(L.sn), D(L.sa), I(L.sperm));
   hipLaunchKernelGGL(k_na_boxes, dim3((L.nsub + 255) / 256), blk, 0, s, ns, L.nsp, L.nsub, D(L.sx), D(L.sub_box), 6);
   Coefs c;
-  target_coefs(HOR_DL_ON, true, 0.0, DEL, c.q);
+  target_coefs(HOR_DL_ON, true, 0.0, DEL, c.c);
   hipLaunchKernelGGL(k_na_targets, dim3((L.ntp + 255) / 256), blk, 0, s, nt, L.ntp, tx_dev, c, D(L.tx), D(L.tcoef),
                      I(L.tnear), D(L.xi), I(L.tperm));
   const int nw = L.ntp / kWave;

@@ -654,8 +654,13 @@ __device__ __forceinline__ void load_target(const DirectArgs& A, int i, Tgt& t) 
   t.py = A.tx[ntp + i];
   t.pz = A.tx[2 * ntp + i];
   t.nearp = A.tnear[i] != 0;
+  if (KIND != HOR_DL_ON && t.nearp) {
+    const double c[3] = {A.tcoef[i], A.tcoef[ntp + i], A.tcoef[2 * ntp + i]};
+    poly_coefs(c, t.q);
+  } else {
 #pragma unroll
-  for (int k = 0; k < kNumCoef; k++) t.q[k] = A.tcoef[k * ntp + i];
+    for (int k = 0; k < kNumCoef; k++) t.q[k] = 0.0;
+  }
   t.sub = 0;
   t.g0x = t.g0y = t.g0z = 0;
   t.x0x = t.x0y = t.x0z = t.n0x = t.n0y = t.n0z = t.b = 0;

@@ -45,7 +45,7 @@ struct hor_plan {
 namespace hor {
 
 bool target_is_near(const hor_plan* P, double b);
-void target_coefs(int kind, bool nearp, double b, double DEL, double q[kNumCoef]);
+void target_coefs(int kind, bool nearp, double b, double DEL, double c[3]);
 int setup_targets(hor_plan* P, const double* tx, const double* tb, const double* tx0, const double* tn0,
                   const double* sx_orig);
 int setup_sources(hor_plan* P, const std::vector<int>& perm, const double* sx, const double* sn,

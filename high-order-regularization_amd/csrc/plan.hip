@@ -119,20 +119,19 @@ bool target_is_near(const hor_plan* P, double b) {
   return !(std::fabs(b) / DEL > 8.0);  // stresslets.cpp:41, KITC.cpp:856, 970
 }
 
-// Per-target polynomial coefficients (see device.h).
-void target_coefs(int kind, bool nearp, double b, double DEL, double q[kNumCoef]) {
-  for (int k = 0; k < kNumCoef; k++) q[k] = 0.0;
-  if (!nearp) return;
+// Per-target regularization coefficients c1, c2, c3 (see device.h; the sum
+// kernels expand them with poly_coefs).  Zero for a target beyond 8 delta and
+// for the on-surface stresslet, whose kernel comes from its own table.
+void target_coefs(int kind, bool nearp, double b, double DEL, double c[3]) {
+  c[0] = c[1] = c[2] = 0.0;
+  if (!nearp || kind == HOR_DL_ON) return;
   if (kind == HOR_SL_ON) {
     const double rp = 1.7724538509055160273;
-    double c[3] = {22.0 / 5.0 / rp, 8.0 / 5.0 / rp, 2.0 / 15.0 / rp};
-    poly_coefs(c, q);
-  } else if (kind == HOR_DL_ON) {
-    poly_coefs_dl_on(q);
+    c[0] = 22.0 / 5.0 / rp;
+    c[1] = 8.0 / 5.0 / rp;
+    c[2] = 2.0 / 15.0 / rp;
   } else {
-    double c[3];
     reg_coeff_7ord(b / DEL, c);
-    poly_coefs(c, q);
   }
 }
 
@@ -148,7 +147,7 @@ int setup_targets(hor_plan* P, const double* tx, const double* tb, const double*
   P->tperm_h = spatial_order(nt, tx, cls.data());
   int ntp = std::max(kBlock, round_up(nt, kBlock));
   P->ntp = ntp;
-  std::vector<double> X(3 * (size_t)ntp), C((size_t)kNumCoef * ntp, 0.0), X0, N0, Bv, XI;
+  std::vector<double> X(3 * (size_t)ntp), C(3 * (size_t)ntp, 0.0), X0, N0, Bv, XI;
   std::vector<int> NEAR(ntp);
   bool off_dl = (kind == HOR_DL_OFF || kind == HOR_SLDL_OFF);
   if (off_dl) {
@@ -163,9 +162,9 @@ int setup_targets(hor_plan* P, const double* tx, const double* tb, const double*
     NEAR[i] = nearp ? 1 : 0;
     for (int k = 0; k < 3; k++) X[(size_t)k * ntp + i] = tx[3 * o + k];
     double b = tb ? tb[o] : 0.0;
-    double q[kNumCoef];
-    target_coefs(kind, nearp, b, P->DEL, q);
-    for (int k = 0; k < kNumCoef; k++) C[(size_t)k * ntp + i] = q[k];
+    double c[3];
+    target_coefs(kind, nearp, b, P->DEL, c);
+    for (int k = 0; k < 3; k++) C[(size_t)k * ntp + i] = c[k];
     if (off_dl) {
       for (int k = 0; k < 3; k++) {
         X0[(size_t)k * ntp + i] = tx0[3 * o + k];
