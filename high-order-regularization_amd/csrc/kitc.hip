@@ -501,15 +501,14 @@ __global__ __launch_bounds__(kBlock / T) void k_far(FarArgs A) {
 #pragma unroll
             for (int q = 0; q < T; q++) {
               const double d2 = pz[q] - t3[c];
-              const double ri = rsqrt64(__builtin_fma(d2, d2, r2ab[q]));
-              const double ri2 = ri * ri;
+              const double ri5 = rsqrt5_64(__builtin_fma(d2, d2, r2ab[q]));
               // d^T M d = M00 d0^2 + M11 d1^2 + S01 d0 d1 + d2 (S02 d0 + S12 d1 + M22 d2)
               const double qab = __builtin_fma(m[3], d01[q], __builtin_fma(m[1], d11[q], m[0] * d00[q]));
               const double w = __builtin_fma(m[2], d2, __builtin_fma(m[5], d1[q], m[4] * d0[q]));
               const double wnd = m[6] * d0[q] + m[7] * d1[q] + m[8] * d2;
               const double gd = __builtin_fma(g0z[q], d2, gab[q]);
               const double Q = __builtin_fma(-gd, wnd, __builtin_fma(w, d2, qab));
-              const double kern = Q * (ri2 * ri2 * ri);
+              const double kern = Q * ri5;
               kab[q] += kern;
               u2[q] += d2 * kern;
             }

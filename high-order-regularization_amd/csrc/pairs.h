@@ -155,11 +155,9 @@ __device__ __forceinline__ void dl_plain(const SrcRec& s, double px, double py, 
                                          double g0y, double g0z, double u[3]) {
   double dx = px - s.x, dy = py - s.y, dz = pz - s.z;
   double r2 = dx * dx + dy * dy + dz * dz;
-  double ri = rsqrt64(r2);
-  double ri2 = ri * ri;
   double gdx = (s.d0 - g0x) * dx + (s.d1 - g0y) * dy + (s.d2 - g0z) * dz;
   double ndx = s.nx * dx + s.ny * dy + s.nz * dz;
-  double k = gdx * ndx * (ri2 * ri2 * ri);
+  double k = gdx * ndx * rsqrt5_64(r2);
   u[0] += dx * k;
   u[1] += dy * k;
   u[2] += dz * k;
@@ -350,9 +348,7 @@ __device__ __forceinline__ void dl_on_full(const SrcRec& s, const Tgt& t, double
   const double sq = r2 * inv_del2;
   double H3;
   if (sq > 64.0) {
-    const double ri = rsqrt64(r2);
-    const double ri2 = ri * ri;
-    H3 = ri2 * ri2 * ri;
+    H3 = rsqrt5_64(r2);
   } else {
     H3 = dl_on_G(tab, sq) * inv_del5;
   }
@@ -607,7 +603,6 @@ __device__ __forceinline__ void make_tgtp(const Tgt& t, const double c0[3], TgtP
 __device__ __forceinline__ void p_plain(const SrcRecP& s, const Tgt& t, const TgtP& v, double u[3]) {
   const double dx = t.px - s.x, dy = t.py - s.y, dz = t.pz - s.z;
   const double r2 = dx * dx + dy * dy + dz * dz;
-  const double ri = rsqrt64(r2);
   double a = v.qx * s.nx;
   a = __builtin_fma(v.qy, s.ny, a);
   a = __builtin_fma(v.qz, s.nz, a);
@@ -619,7 +614,7 @@ __device__ __forceinline__ void p_plain(const SrcRecP& s, const Tgt& t, const Tg
   a = __builtin_fma(-t.g0z, s.m2, a);
   a = __builtin_fma(-t.sub, s.al, a);
   a = a + s.be;
-  u[0] += a * (ri * ri * ri);
+  u[0] += a * rsqrt3_64(r2);
 }
 
 // Full branch structure of Lapl_DL_7ord (extend_to_grid/utilities.cpp:190-211):
