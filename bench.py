@@ -408,7 +408,7 @@ def tree_roofline(S, phase, freg, mixed, counted=None):
     near_flops = near * (freg * OPS["dl_on_reg"] + (1 - freg) * OPS["dl_plain"])
     mom_flops = mom * OPS["dl_moment"]
     out = {
-        "far": {"kernel": "k_far<true, %s>" % ("true" if mixed else "false"), "bound": "valu",
+        "far": {"kernel": "k_far<true, true, 2>" if mixed else "k_far<true, false, 1>", "bound": "valu",
                 "compute": "FP32 vector ALU, FP64 row sums" if mixed else "FP64 vector ALU",
                 "ms": phase["far_ms"], "flop": far_flops, "achieved": far_flops / (phase["far_ms"] * 1e-3) / 1e12,
                 "peak": peak_far / 1e12, "unit": "TFLOP/s",

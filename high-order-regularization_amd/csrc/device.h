@@ -125,24 +125,37 @@ __device__ __forceinline__ double rsqrt64(double x) {
   return __builtin_fma(y * e, __builtin_fma(e, 0.375, 0.5), y);
 }
 
+// An FP64 constant kept in vector registers for a loop.  A VALU instruction
+// of this ISA reads at most one scalar register and no 64-bit literal, so an
+// FMA with two non-inline constants needs one of them in vector registers;
+// the empty asm keeps the compiler from rebuilding it (two moves) at every use.
+__device__ __forceinline__ double vreg(double v) {
+  asm("" : "+v"(v));
+  return v;
+}
+
 // x^(-3/2) and x^(-5/2) directly from the hardware estimate y of x^(-1/2):
 // with e = 1 - x y^2 (|e| < 1.1e-7), x^(-k/2) = y^k (1 - e)^(-k/2) and the
 // binomial series to e^2, (1 + 3/2 e + 15/8 e^2) and (1 + 5/2 e + 35/8 e^2);
 // the e^3 term is below 1e-20 relative, and each costs one instruction fewer
-// than 1/sqrt(x) refined and raised to the power.
-__device__ __forceinline__ double rsqrt3_64(double x) {
+// than 1/sqrt(x) refined and raised to the power.  c2 is the e^2 coefficient
+// (15/8 or 35/8) held in vector registers by the caller (vreg, outside its
+// loops): with the e coefficient as the FMA's scalar addend the compiler
+// emits one three-address FMA, where a vector-register addend would be tied
+// to the result and copied at every use.
+__device__ __forceinline__ double rsqrt3_64(double x, double c2) {
   const double y = __builtin_amdgcn_rsq(x);
   const double y2 = y * y;
   const double e = __builtin_fma(-x, y2, 1.0);
   const double y3 = y2 * y;
-  return __builtin_fma(y3 * e, __builtin_fma(e, 1.875, 1.5), y3);
+  return __builtin_fma(y3 * e, __builtin_fma(e, c2, 1.5), y3);
 }
-__device__ __forceinline__ double rsqrt5_64(double x) {
+__device__ __forceinline__ double rsqrt5_64(double x, double c2) {
   const double y = __builtin_amdgcn_rsq(x);
   const double y2 = y * y;
   const double e = __builtin_fma(-x, y2, 1.0);
   const double y5 = y2 * y2 * y;
-  return __builtin_fma(y5 * e, __builtin_fma(e, 4.375, 2.5), y5);
+  return __builtin_fma(y5 * e, __builtin_fma(e, c2, 2.5), y5);
 }
 
 __device__ __forceinline__ float rsqrt32(float x) { return __builtin_amdgcn_rsqf(x); }

@@ -470,6 +470,7 @@ __global__ __launch_bounds__(kBlock / T) void k_far(FarArgs A) {
         }
       }
     } else {
+      const double k5 = vreg(4.375);   // rsqrt5_64's e^2 coefficient, in vector registers for the loops
       double g0x[T], g0y[T], g0z[T];
 #pragma unroll
       for (int q = 0; q < T; q++) {
@@ -501,7 +502,7 @@ __global__ __launch_bounds__(kBlock / T) void k_far(FarArgs A) {
 #pragma unroll
             for (int q = 0; q < T; q++) {
               const double d2 = pz[q] - t3[c];
-              const double ri5 = rsqrt5_64(__builtin_fma(d2, d2, r2ab[q]));
+              const double ri5 = rsqrt5_64(__builtin_fma(d2, d2, r2ab[q]), k5);
               // d^T M d = M00 d0^2 + M11 d1^2 + S01 d0 d1 + d2 (S02 d0 + S12 d1 + M22 d2)
               const double qab = __builtin_fma(m[3], d01[q], __builtin_fma(m[1], d11[q], m[0] * d00[q]));
               const double w = __builtin_fma(m[2], d2, __builtin_fma(m[5], d1[q], m[4] * d0[q]));

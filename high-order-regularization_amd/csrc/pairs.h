@@ -152,12 +152,12 @@ __device__ __forceinline__ void sl_full(const SrcRec& s, double px, double py, d
 // subtraction form (stresslets.cpp:157-167) evaluated in the algebraically
 // identical, cancellation-free product form.
 __device__ __forceinline__ void dl_plain(const SrcRec& s, double px, double py, double pz, double g0x,
-                                         double g0y, double g0z, double u[3]) {
+                                         double g0y, double g0z, double k5, double u[3]) {
   double dx = px - s.x, dy = py - s.y, dz = pz - s.z;
   double r2 = dx * dx + dy * dy + dz * dz;
   double gdx = (s.d0 - g0x) * dx + (s.d1 - g0y) * dy + (s.d2 - g0z) * dz;
   double ndx = s.nx * dx + s.ny * dy + s.nz * dz;
-  double k = gdx * ndx * rsqrt5_64(r2);
+  double k = gdx * ndx * rsqrt5_64(r2, k5);
   u[0] += dx * k;
   u[1] += dy * k;
   u[2] += dz * k;
@@ -295,9 +295,11 @@ __device__ __forceinline__ void sldl_full(const SrcRecV& s, const Tgt& t, double
       const double p2 = q[3] + sq * (q[4] + sq * (q[5] + sq * q[6]));
       const double p3 = q[7] + sq * (q[8] + sq * (q[9] + sq * (q[10] + sq * q[11])));
       s1 = (A + p1 * E) * inv_del;
-      s2 = (A + p2 * E) * inv_del * (ri * ri);
+      // the Stokeslet's s2 and the stresslet's H2 are one function (Stokes_SL_7ord
+      // and Stokes_DL_7ord, utilities.cpp:199-205, 276-285)
       const double w2 = inv_del * ri2;
-      H2 = (A + p2 * E) * w2;
+      s2 = (A + p2 * E) * w2;
+      H2 = s2;
       H3 = (A + p3 * E) * (w2 * ri2);
     }
   }
@@ -341,14 +343,14 @@ __device__ __forceinline__ void sldl_full(const SrcRecV& s, const Tgt& t, double
 // H3 = G(rho^2)/delta^5 from the table (no exp, erf or square root), beyond
 // it the plain 1/r^5.
 __device__ __forceinline__ void dl_on_full(const SrcRec& s, const Tgt& t, double inv_del2, double inv_del5,
-                                           const double* __restrict__ tab, double u[3]) {
+                                           const double* __restrict__ tab, double k5, double u[3]) {
   double dx = t.px - s.x, dy = t.py - s.y, dz = t.pz - s.z;
   double r2 = dx * dx + dy * dy + dz * dz;
   if (r2 < 1e-28) return;  // d = 0: the self term contributes exactly zero
   const double sq = r2 * inv_del2;
   double H3;
   if (sq > 64.0) {
-    H3 = rsqrt5_64(r2);
+    H3 = rsqrt5_64(r2, k5);
   } else {
     H3 = dl_on_G(tab, sq) * inv_del5;
   }
@@ -600,7 +602,7 @@ __device__ __forceinline__ void make_tgtp(const Tgt& t, const double c0[3], TgtP
 // precomputed al, be, m.  Its terms are of size |p'| where the numerator is
 // of size |d| > 8 delta, so the chain's rounding stays within a few ulps of
 // the numerator at the distances of plain pairs.
-__device__ __forceinline__ void p_plain(const SrcRecP& s, const Tgt& t, const TgtP& v, double u[3]) {
+__device__ __forceinline__ void p_plain(const SrcRecP& s, const Tgt& t, const TgtP& v, double k3, double u[3]) {
   const double dx = t.px - s.x, dy = t.py - s.y, dz = t.pz - s.z;
   const double r2 = dx * dx + dy * dy + dz * dz;
   double a = v.qx * s.nx;
@@ -614,7 +616,7 @@ __device__ __forceinline__ void p_plain(const SrcRecP& s, const Tgt& t, const Tg
   a = __builtin_fma(-t.g0z, s.m2, a);
   a = __builtin_fma(-t.sub, s.al, a);
   a = a + s.be;
-  u[0] += a * rsqrt3_64(r2);
+  u[0] += a * rsqrt3_64(r2, k3);
 }
 
 // Full branch structure of Lapl_DL_7ord (extend_to_grid/utilities.cpp:190-211):
@@ -706,6 +708,7 @@ __device__ __forceinline__ void range_sum(const DirectArgs& A, int j0, int j1, i
   const SrcRec* __restrict__ R = A.rec;
   const SrcRec32* __restrict__ R32 = A.rec32;
   constexpr bool SL = (KIND == HOR_SL_ON || KIND == HOR_SL_OFF);
+  const double k5 = vreg(4.375);   // rsqrt5_64's e^2 coefficient, in vector registers for the loops
   Tgt32 v;
   if (MIXED) make_tgt32(t, v);
   const float inv_del32 = (float)inv_del;
@@ -732,7 +735,7 @@ __device__ __forceinline__ void range_sum(const DirectArgs& A, int j0, int j1, i
 #pragma unroll 2
         for (int j = base; j < end; j++) {
           if (SL) sl_plain(R[j], t.px, t.py, t.pz, t.sub, u);
-          else dl_plain(R[j], t.px, t.py, t.pz, t.g0x, t.g0y, t.g0z, u);
+          else dl_plain(R[j], t.px, t.py, t.pz, t.g0x, t.g0y, t.g0z, k5, u);
         }
       }
     } else {
@@ -755,10 +758,10 @@ __device__ __forceinline__ void range_sum(const DirectArgs& A, int j0, int j1, i
         } else if (__any(need ? 1 : 0)) {
           if (SL) sl_full(s, t.px, t.py, t.pz, t.sub, t.nearp, t.q, inv_del, inv_del2, self1, self2, tab, u);
           else if (KIND == HOR_DL_OFF) dl_off_full(s, t, inv_del, inv_del2, self2, tab, u);
-          else dl_on_full(s, t, inv_del2, inv_del5, tab, u);
+          else dl_on_full(s, t, inv_del2, inv_del5, tab, k5, u);
         } else {
           if (SL) sl_plain(s, t.px, t.py, t.pz, t.sub, u);
-          else dl_plain(s, t.px, t.py, t.pz, t.g0x, t.g0y, t.g0z, u);
+          else dl_plain(s, t.px, t.py, t.pz, t.g0x, t.g0y, t.g0z, k5, u);
         }
       }
       if (MIXED) {
@@ -780,6 +783,7 @@ __device__ __forceinline__ void range_sum_p(const DirectArgs& A, int j0, int j1,
   const double inv_del2 = inv_del * inv_del;
   const double self2 = 256.0 / 15.0 / rp / (A.DEL * A.DEL * A.DEL);
   const SrcRecP* __restrict__ R = A.recp;
+  const double k3 = vreg(1.875);   // rsqrt3_64's e^2 coefficient, in vector registers for the loops
   TgtP v;
   make_tgtp(t, A.c0, v);
   for (int base = j0, sb = sub0; base < j1; base += kWave, sb++) {
@@ -787,7 +791,7 @@ __device__ __forceinline__ void range_sum_p(const DirectArgs& A, int j0, int j1,
     const double* box = A.sub_box + 6 * sb;
     if (all_far || boxes_far(wb, box, A.far_d2)) {
 #pragma unroll 2
-      for (int j = base; j < end; j++) p_plain(R[j], t, v, u);
+      for (int j = base; j < end; j++) p_plain(R[j], t, v, k3, u);
     } else {
       for (int j = base; j < end; j++) {
         const SrcRecP s = R[j];
@@ -795,7 +799,7 @@ __device__ __forceinline__ void range_sum_p(const DirectArgs& A, int j0, int j1,
         const double r2 = dx * dx + dy * dy + dz * dz;
         const bool need = t.nearp && !(r2 > A.near_r2);
         if (__any(need ? 1 : 0)) p_full(s, t, inv_del, inv_del2, self2, tab, u);
-        else p_plain(s, t, v, u);
+        else p_plain(s, t, v, k3, u);
       }
     }
   }
