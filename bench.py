@@ -699,7 +699,7 @@ def main():
     kp_ms = float(np.mean(kp))
     p_flops = ns * nt * (fnear * OPS["p_reg"] + (1 - fnear) * OPS["p_plain"])
     pressure = {"ms_per_eval": p_ms, "value": ns * nt / (p_ms * 1e-3), "unit": "pair interactions/s",
-                "kernel": "k_direct<P_OFF,FP64>", "avg_kernel_ms": kp_ms,
+                "kernel": "k_pressure", "avg_kernel_ms": kp_ms,
                 "achieved_tflops": p_flops / (kp_ms * 1e-3) / 1e12,
                 "frac": p_flops / (kp_ms * 1e-3) / PEAK_FP64}
     pp.close()

@@ -196,15 +196,15 @@ struct __align__(16) SrcRecV32 {
   float f2, g0, g1, g2;
 };
 
-// Record of the regularized pressure (HOR_P_OFF): position, a n, a f and,
-// relative to the plan's source-box centre c0 (x' = x - c0), al = a n.x',
-// be = a f.x' and m = a n x x', which turn the plain pair's numerator into one
-// chain of fused multiply-adds (p_plain).
+// Record of the regularized pressure (HOR_P_OFF): position x' = x - c relative
+// to the centre c of the source's 64-subtile box, a n, a f, al = a n.x',
+// be = a f.x', m = a n x x' and xx = |x'|^2, which turn the plain pair's r^2
+// and numerator into chains of fused multiply-adds (p_plain).
 struct __align__(16) SrcRecP {
   double x, y, z, al;
   double nx, ny, nz, be;
   double d0, d1, d2, m0;
-  double m1, m2, pad0, pad1;
+  double m1, m2, xx, pad;
 };
 
 struct __align__(16) SrcRec32 {

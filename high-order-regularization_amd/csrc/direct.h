@@ -22,7 +22,6 @@ struct DirectArgs {
   const double* sd;                // 3*nsp density (f or g)
   const struct SrcRec* rec;        // nsp source records (sorted), this evaluation's density
   const struct SrcRecP* recp;      // P_OFF: nsp pressure records
-  double c0[3];                    // P_OFF: the centre the records' al, be, m refer to
   const struct SrcRecV* recv;      // SLDL_OFF: nsp fused records
   const struct SrcRecV32* recv32;  // SLDL_OFF mixed precision: FP32 copies (y - subtile centre)
   const struct SrcRec32* rec32;    // FP32 copies (mixed precision) or null; direct plans store
@@ -56,9 +55,9 @@ int launch_gather3(const double* dens, const int* perm, int n, int np, const dou
                    const double* sub_box, int scale_density, hipStream_t s, const int* sub_of = nullptr);
 int launch_sldl_finalize(const double* partial, int nchunk, int nt, int ntp, double scale_sl, double scale_dl,
                          const double* xi, const double* g0, const int* tperm, double* out, hipStream_t s);
-// Pressure records (SrcRecP) with al, be, m relative to c0.
+// Pressure records (SrcRecP), positions relative to their subtile's box centre.
 int launch_gather_p(const double* dens, const int* perm, int n, int np, const double* sx, const double* sn,
-                    const double* sa, const double c0[3], struct SrcRecP* recp, hipStream_t s);
+                    const double* sa, const double* sub_box, struct SrcRecP* recp, hipStream_t s);
 int launch_gather_sldl(const double* dens, const int* perm, int n, int np, const double* sx, const double* sn,
                        const double* sa, struct SrcRecV* recv, struct SrcRecV32* recv32, const double* sub_box,
                        hipStream_t s);

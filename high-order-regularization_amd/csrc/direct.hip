@@ -37,16 +37,29 @@ __global__ __launch_bounds__(kBlock) void k_direct(DirectArgs A) {
   // this chunk's 64-source subtiles
   const int sub0 = blockIdx.y * A.subs_per_chunk;
   const int sub1 = min(A.nsub, sub0 + A.subs_per_chunk);
-  if constexpr (KIND == HOR_P_OFF)
-    range_sum_p(A, sub0 * kWave, min(A.ns, sub1 * kWave), sub0, wb, all_far, t, tab, u);
-  else
-    range_sum<KIND, MIXED, true>(A, sub0 * kWave, min(A.ns, sub1 * kWave), sub0, wb, all_far, t, tab, u);
+  range_sum<KIND, MIXED, true>(A, sub0 * kWave, min(A.ns, sub1 * kWave), sub0, wb, all_far, t, tab, u);
   double* P = A.partial + (size_t)blockIdx.y * 3 * A.ntp;
   P[i] = u[0];
-  if (KIND != HOR_P_OFF) {
-    P[A.ntp + i] = u[1];
-    P[2 * A.ntp + i] = u[2];
-  }
+  P[A.ntp + i] = u[1];
+  P[2 * A.ntp + i] = u[2];
+}
+
+// The regularized pressure (HOR_P_OFF, FP64), one value per target and
+// chunk.  Launch bounds for five waves per SIMD (at most 96 VGPRs): the
+// per-subtile target data of range_sum_p would otherwise take 103 VGPRs and
+// four waves, 3% slower on the band workload.
+__global__ __launch_bounds__(kBlock, 5) void k_pressure(DirectArgs A) {
+  const double* tab = stage_table<HOR_P_OFF>(A);
+  const int i = blockIdx.x * kBlock + threadIdx.x;   // sorted target (ntp is a multiple of kBlock)
+  Tgt t;
+  load_target<HOR_P_OFF>(A, i, t);
+  const double* wb = A.wave_box + 8 * (i / kWave);
+  const bool all_far = wb[6] != 0.0;
+  double u[3] = {0.0, 0.0, 0.0};
+  const int sub0 = blockIdx.y * A.subs_per_chunk;
+  const int sub1 = min(A.nsub, sub0 + A.subs_per_chunk);
+  range_sum_p(A, sub0 * kWave, min(A.ns, sub1 * kWave), sub0, wb, all_far, t, tab, u);
+  A.partial[(size_t)blockIdx.y * 3 * A.ntp + i] = u[0];
 }
 
 // Stokeslet + stresslet at off-surface targets in one pass (HOR_SLDL_OFF):
@@ -172,25 +185,29 @@ __global__ void k_gather3(const double* dens, const int* perm, int n, int np, co
   }
 }
 
-// Pressure records: recp[j] <- (y, a n.y', a n, a f.y', a f, a n x y') of
-// sorted source j with y' = y - c0, dens in the caller's order.
+// Pressure records: recp[j] <- (y', a n.y', a n, a f.y', a f, a n x y', |y'|^2)
+// of sorted source j with y' = y - c, c the centre of its 64-subtile box,
+// dens in the caller's order.
 __global__ void k_gather_p(const double* dens, const int* perm, int n, int np, const double* sx, const double* sn,
-                           const double* sa, double c0x, double c0y, double c0z, SrcRecP* recp) {
+                           const double* sa, const double* sub_box, SrcRecP* recp) {
   int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
   int o = perm[j];
   const double a = sa[j];
+  const double* bx = sub_box + 6 * (j / kWave);
   SrcRecP r;
-  r.x = sx[j]; r.y = sx[np + j]; r.z = sx[2 * np + j];
+  r.x = sx[j] - 0.5 * (bx[0] + bx[3]);
+  r.y = sx[np + j] - 0.5 * (bx[1] + bx[4]);
+  r.z = sx[2 * np + j] - 0.5 * (bx[2] + bx[5]);
   r.nx = a * sn[j]; r.ny = a * sn[np + j]; r.nz = a * sn[2 * np + j];
   r.d0 = a * dens[3 * o]; r.d1 = a * dens[3 * o + 1]; r.d2 = a * dens[3 * o + 2];
-  const double yx = r.x - c0x, yy = r.y - c0y, yz = r.z - c0z;
-  r.al = r.nx * yx + r.ny * yy + r.nz * yz;
-  r.be = r.d0 * yx + r.d1 * yy + r.d2 * yz;
-  r.m0 = r.ny * yz - r.nz * yy;
-  r.m1 = r.nz * yx - r.nx * yz;
-  r.m2 = r.nx * yy - r.ny * yx;
-  r.pad0 = r.pad1 = 0.0;
+  r.al = r.nx * r.x + r.ny * r.y + r.nz * r.z;
+  r.be = r.d0 * r.x + r.d1 * r.y + r.d2 * r.z;
+  r.m0 = r.ny * r.z - r.nz * r.y;
+  r.m1 = r.nz * r.x - r.nx * r.z;
+  r.m2 = r.nx * r.y - r.ny * r.x;
+  r.xx = r.x * r.x + r.y * r.y + r.z * r.z;
+  r.pad = 0.0;
   recp[j] = r;
 }
 
@@ -300,7 +317,7 @@ int launch_direct(const DirectArgs& A, int kind, bool mixed, int nchunk, hipStre
       case HOR_SL_ON: HOR_LAUNCH(HOR_SL_ON, false); break;
       case HOR_SL_OFF: HOR_LAUNCH(HOR_SL_OFF, false); break;
       case HOR_DL_ON: HOR_LAUNCH(HOR_DL_ON, false); break;
-      case HOR_P_OFF: HOR_LAUNCH(HOR_P_OFF, false); break;
+      case HOR_P_OFF: hipLaunchKernelGGL(k_pressure, grid, block, lds_bytes(HOR_P_OFF), s, A); break;
       default: HOR_LAUNCH(HOR_DL_OFF, false); break;
     }
   } else {
@@ -346,10 +363,10 @@ int launch_gather_sldl(const double* dens, const int* perm, int n, int np, const
 }
 
 int launch_gather_p(const double* dens, const int* perm, int n, int np, const double* sx, const double* sn,
-                    const double* sa, const double c0[3], SrcRecP* recp, hipStream_t s) {
+                    const double* sa, const double* sub_box, SrcRecP* recp, hipStream_t s) {
   if (n == 0) return HOR_OK;
-  hipLaunchKernelGGL(k_gather_p, dim3((n + 255) / 256), dim3(256), 0, s, dens, perm, n, np, sx, sn, sa, c0[0], c0[1],
-                     c0[2], recp);
+  hipLaunchKernelGGL(k_gather_p, dim3((n + 255) / 256), dim3(256), 0, s, dens, perm, n, np, sx, sn, sa, sub_box,
+                     recp);
   HOR_HIP_TRY(hipGetLastError());
   return HOR_OK;
 }

@@ -581,30 +581,36 @@ __device__ __forceinline__ double p_numer(const Rec& s, const Tgt& t, double dx,
   return (t.sub * nd - fd) + nc;
 }
 
-// Per-target data of the pressure's plain pairs: p' = p - c0 and
+// Per-target data of the pressure's plain pairs in the frame of one source
+// subtile (box centre c): p' = p - c, m2 = -2 p', pp = |p'|^2 and
 // q = (f0.n0) p' + p' x (n0 x f0).
 struct TgtP {
-  double px, py, pz, qx, qy, qz;
+  double px, py, pz, qx, qy, qz, m2x, m2y, m2z, pp;
 };
 
-__device__ __forceinline__ void make_tgtp(const Tgt& t, const double c0[3], TgtP& v) {
-  v.px = t.px - c0[0];
-  v.py = t.py - c0[1];
-  v.pz = t.pz - c0[2];
+__device__ __forceinline__ void make_tgtp(const Tgt& t, const double* box, TgtP& v) {
+  v.px = t.px - 0.5 * (box[0] + box[3]);
+  v.py = t.py - 0.5 * (box[1] + box[4]);
+  v.pz = t.pz - 0.5 * (box[2] + box[5]);
   v.qx = t.sub * v.px + (v.py * t.g0z - v.pz * t.g0y);
   v.qy = t.sub * v.py + (v.pz * t.g0x - v.px * t.g0z);
   v.qz = t.sub * v.pz + (v.px * t.g0y - v.py * t.g0x);
+  v.m2x = -2.0 * v.px;
+  v.m2y = -2.0 * v.py;
+  v.m2z = -2.0 * v.pz;
+  v.pp = v.px * v.px + v.py * v.py + v.pz * v.pz;
 }
 
-// Plain kernel H2 = 1/r^3 (Lapl_DL_7ord beyond 8 delta).  With d = p' - x'
-// (x' = x - c0) the numerator (f0.n0)(n.d) - f.d + n.(d x c), c = n0 x f0,
-// is n.q - f.p' - c.m + (be - (f0.n0) al): one FMA chain on the record's
-// precomputed al, be, m.  Its terms are of size |p'| where the numerator is
-// of size |d| > 8 delta, so the chain's rounding stays within a few ulps of
-// the numerator at the distances of plain pairs.
-__device__ __forceinline__ void p_plain(const SrcRecP& s, const Tgt& t, const TgtP& v, double k3, double u[3]) {
-  const double dx = t.px - s.x, dy = t.py - s.y, dz = t.pz - s.z;
-  const double r2 = dx * dx + dy * dy + dz * dz;
+// Plain kernel H2 = 1/r^3 (Lapl_DL_7ord beyond 8 delta), everything in the
+// subtile's frame (d = p' - x').  r^2 = (|p'|^2 + |x'|^2) - 2 p'.x' is one
+// FMA chain on the record's xx; the numerator (f0.n0)(n.d) - f.d + n.(d x c),
+// c = n0 x f0, is n.q - f.p' - c.m + (be - (f0.n0) al), one FMA chain on the
+// record's precomputed al, be, m.  The terms of both are of size |p'| + |x'|,
+// where |x'| is within the subtile's box and |p'| - |x'| <= r with r > 8 delta
+// for every plain pair, so their rounding stays within a few ulps of r^2 and
+// of the numerator.
+__device__ __forceinline__ void p_plain_r2(const SrcRecP& s, const Tgt& t, const TgtP& v, double r2, double k3,
+                                           double u[3]) {
   double a = v.qx * s.nx;
   a = __builtin_fma(v.qy, s.ny, a);
   a = __builtin_fma(v.qz, s.nz, a);
@@ -619,13 +625,21 @@ __device__ __forceinline__ void p_plain(const SrcRecP& s, const Tgt& t, const Tg
   u[0] += a * rsqrt3_64(r2, k3);
 }
 
-// Full branch structure of Lapl_DL_7ord (extend_to_grid/utilities.cpp:190-211):
-// self limit, plain, or (erf + P2(rho^2) rho e^{-rho^2}) / r^3 with the
-// Stokes_SL_7ord s2 polynomial (the same coefficients q[3..6]).
-__device__ __forceinline__ void p_full(const SrcRecP& s, const Tgt& t, double inv_del, double inv_del2,
-                                       double self2, const double* __restrict__ tab, double u[3]) {
-  const double dx = t.px - s.x, dy = t.py - s.y, dz = t.pz - s.z;
-  const double r2 = dx * dx + dy * dy + dz * dz;
+__device__ __forceinline__ void p_plain(const SrcRecP& s, const Tgt& t, const TgtP& v, double k3, double u[3]) {
+  double r2 = v.pp + s.xx;
+  r2 = __builtin_fma(v.m2x, s.x, r2);
+  r2 = __builtin_fma(v.m2y, s.y, r2);
+  r2 = __builtin_fma(v.m2z, s.z, r2);
+  p_plain_r2(s, t, v, r2, k3, u);
+}
+
+// Full branch structure of Lapl_DL_7ord (extend_to_grid/utilities.cpp:190-211)
+// at a pair of a mixed subtile, d = p' - x' and r^2 from the vote: self limit,
+// plain, or (erf + P2(rho^2) rho e^{-rho^2}) / r^3 with the Stokes_SL_7ord s2
+// polynomial (the same coefficients q[3..6]).
+__device__ __forceinline__ void p_full(const SrcRecP& s, const Tgt& t, double dx, double dy, double dz, double r2,
+                                       double inv_del, double inv_del2, double self2,
+                                       const double* __restrict__ tab, double u[3]) {
   double H2;
   if (t.nearp && r2 < 1e-28) {
     H2 = self2;
@@ -774,7 +788,9 @@ __device__ __forceinline__ void range_sum(const DirectArgs& A, int j0, int j1, i
 }
 
 // Regularized pressure over the sorted sources [j0, j1) (HOR_P_OFF, FP64):
-// the subtile test and per-source vote of range_sum on the pressure records.
+// the subtile test and per-source vote of range_sum on the pressure records,
+// in each subtile's frame (the records' positions are relative to its box
+// centre).
 __device__ __forceinline__ void range_sum_p(const DirectArgs& A, int j0, int j1, int sub0, const double* wb,
                                             bool all_far, const Tgt& t, const double* __restrict__ tab,
                                             double u[3]) {
@@ -784,22 +800,22 @@ __device__ __forceinline__ void range_sum_p(const DirectArgs& A, int j0, int j1,
   const double self2 = 256.0 / 15.0 / rp / (A.DEL * A.DEL * A.DEL);
   const SrcRecP* __restrict__ R = A.recp;
   const double k3 = vreg(1.875);   // rsqrt3_64's e^2 coefficient, in vector registers for the loops
-  TgtP v;
-  make_tgtp(t, A.c0, v);
   for (int base = j0, sb = sub0; base < j1; base += kWave, sb++) {
     const int end = min(j1, base + kWave);
     const double* box = A.sub_box + 6 * sb;
+    TgtP v;
+    make_tgtp(t, box, v);
     if (all_far || boxes_far(wb, box, A.far_d2)) {
 #pragma unroll 2
       for (int j = base; j < end; j++) p_plain(R[j], t, v, k3, u);
     } else {
       for (int j = base; j < end; j++) {
         const SrcRecP s = R[j];
-        const double dx = t.px - s.x, dy = t.py - s.y, dz = t.pz - s.z;
+        const double dx = v.px - s.x, dy = v.py - s.y, dz = v.pz - s.z;
         const double r2 = dx * dx + dy * dy + dz * dz;
         const bool need = t.nearp && !(r2 > A.near_r2);
-        if (__any(need ? 1 : 0)) p_full(s, t, inv_del, inv_del2, self2, tab, u);
-        else p_plain(s, t, v, k3, u);
+        if (__any(need ? 1 : 0)) p_full(s, t, dx, dy, dz, r2, inv_del, inv_del2, self2, tab, u);
+        else p_plain_r2(s, t, v, r2, k3, u);
       }
     }
   }

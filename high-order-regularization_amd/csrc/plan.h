@@ -26,7 +26,6 @@ struct hor_plan {
   hor::DBuf<hor::SrcRecV> recv;  // SLDL_OFF records
   hor::DBuf<hor::SrcRecV32> recv32;
   hor::DBuf<hor::SrcRecP> recp;  // P_OFF records
-  double c0[3] = {0, 0, 0};      // P_OFF: centre of the sources' bounding box
   hor::DBuf<double> dlon_tab;  // on-surface stresslet table (kDlOnTable), DL_ON plans
   hor::DBuf<double> ae_tab;    // erf(rho)/rho and exp(-rho^2) table (kErfExpTable), other plans
   hor::DBuf<int> sperm;

@@ -263,13 +263,6 @@ int setup_sources(hor_plan* P, const std::vector<int>& perm, const double* sx, c
     HOR_TRY(P->recv.alloc(nsp));
     if (P->precision == HOR_MIXED) HOR_TRY(P->recv32.alloc(nsp));
   } else if (P->kind == HOR_P_OFF) {
-    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-    for (int j = 0; j < ns; j++)
-      for (int k = 0; k < 3; k++) {
-        lo[k] = std::min(lo[k], sx[3 * j + k]);
-        hi[k] = std::max(hi[k], sx[3 * j + k]);
-      }
-    for (int k = 0; k < 3; k++) P->c0[k] = 0.5 * (lo[k] + hi[k]);
     HOR_TRY(P->recp.alloc(nsp));
   } else {
     HOR_TRY(P->rec.alloc(nsp));
@@ -293,7 +286,7 @@ DirectArgs direct_args(hor_plan* P) {
   A.near64_r2 = 16.0 * P->DEL * P->DEL;
   A.sx = P->sx.p; A.sn = P->sn.p; A.sa = P->sa.p; A.sd = P->sd.p; A.sub_box = P->sub_box.p;
   A.rec = P->rec.p; A.recv = P->recv.p; A.recp = P->recp.p;
-  for (int k = 0; k < 3; k++) A.c0[k] = P->c0[k]; A.rec32 = P->precision == HOR_MIXED ? P->rec32.p : nullptr;
+  A.rec32 = P->precision == HOR_MIXED ? P->rec32.p : nullptr;
   A.recv32 = P->precision == HOR_MIXED ? P->recv32.p : nullptr;
   A.tx = P->tx.p; A.tcoef = P->tcoef.p; A.tsub = P->tsub.p;
   A.tx0 = P->tx0.p; A.tn0 = P->tn0.p; A.tb = P->tb.p;
@@ -364,7 +357,7 @@ static int plan_eval_direct(hor_plan* P, const double* dens_dev, const double* d
                                 P->xi.p, P->tsub.p + P->ntp, P->tperm.p, out_dev, s);
   }
   if (P->kind == HOR_P_OFF)
-    HOR_TRY(launch_gather_p(dens_dev, P->sperm.p, P->ns, P->nsp, P->sx.p, P->sn.p, P->sa.p, P->c0, P->recp.p, s));
+    HOR_TRY(launch_gather_p(dens_dev, P->sperm.p, P->ns, P->nsp, P->sx.p, P->sn.p, P->sa.p, P->sub_box.p, P->recp.p, s));
   else
     HOR_TRY(launch_gather3(dens_dev, P->sperm.p, P->ns, P->nsp, P->sx.p, P->sn.p, P->sa.p, P->sd.p, P->rec.p,
                            P->precision == HOR_MIXED ? P->rec32.p : nullptr, P->sub_box.p,
