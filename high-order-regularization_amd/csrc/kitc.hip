@@ -75,6 +75,18 @@ struct TreePlan {
   // reduction
   DBuf<int> slot_start;      // sorted target order, nt+1
   DBuf<double> slots;        // 3*nslot
+  // Double layer: the target-side subtraction of every pair term, g - g0 with
+  // g0 the target's density (KITC.cpp:1461, 1678), is linear in g0 with
+  // coefficients that depend only on the geometry:
+  //   sum_j K_ij (g_j - g0_i) = sum_j K_ij g_j - C_i g0_i,
+  // C_i the sum of the near pairs' K_ij (the leaf sums subtract only at a near
+  // target of the off-surface plan) and of the far proxies' (W_n.d) d d^T / r^5.
+  // csub holds scale * C_i, SoA [(3 k + l) ntp + i] in sorted target order,
+  // computed at plan creation by the plan's own FP64 kernels on zero density
+  // with g0 = -e_l (the three columns); every later evaluation runs the
+  // kernels on g alone (NOSUB) and k_tree_finalize subtracts csub g0.
+  DBuf<double> csub;
+  bool has_csub = false;
   // phase events of the last evaluation: before the moments, after the
   // moments, after the near sums, after the far field
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -245,8 +257,9 @@ __device__ __forceinline__ double wave_max(double v) {
 }
 
 // Leaf direct sums (Call_Ds_*, KITC.cpp:807-1062) for up to 256 near targets
-// over one chunk of a leaf's sources.
-template <int KIND, bool MIXED>
+// over one chunk of a leaf's sources.  NOSUB: the double layer on g, not
+// g - g0 (see TreePlan::csub).
+template <int KIND, bool MIXED, bool NOSUB>
 __global__ __launch_bounds__(kBlock) void k_leaf(LeafArgs A) {
   const double* tab = stage_table<KIND>(A.D);   // the workgroup's only barrier
   const int tid = threadIdx.x;
@@ -259,7 +272,7 @@ __global__ __launch_bounds__(kBlock) void k_leaf(LeafArgs A) {
   const int li = off + (act ? tid : 0);
   const int i = A.tgt[li];
   Tgt t;
-  load_target<KIND>(A.D, i, t);
+  load_target<KIND, NOSUB>(A.D, i, t);
   // wave bounding box and class
   double wb[6];
   wb[0] = wave_min(t.px); wb[1] = wave_min(t.py); wb[2] = wave_min(t.pz);
@@ -301,8 +314,9 @@ constexpr int kNc32Dl = 12;   // (9 used)
 // targets (target tid + q * 256/T of the item for q < T): every moment loaded
 // serves T targets and the T chains of arithmetic interleave.  In mixed
 // precision the proxy interactions run in FP32 on node-centred coordinates,
-// each row of P+1 proxies is summed in FP32 and the rows in FP64.
-template <bool DL, bool MIXED, int T>
+// each row of P+1 proxies is summed in FP32 and the rows in FP64.  NOSUB:
+// the stresslet without its g0 term -(g0.d)(W_n.d) (see TreePlan::csub).
+template <bool DL, bool MIXED, int T, bool NOSUB>
 __global__ __launch_bounds__(kBlock / T) void k_far(FarArgs A) {
   constexpr int NT = kBlock / T;           // threads per workgroup
   const int tid = threadIdx.x;
@@ -417,9 +431,12 @@ __global__ __launch_bounds__(kBlock / T) void k_far(FarArgs A) {
               const float ri2 = ri * ri;
               const float qab = __builtin_fmaf(m[3], d01[q], __builtin_fmaf(m[1], d11[q], m[0] * d00[q]));
               const float w = __builtin_fmaf(m[2], d2, __builtin_fmaf(m[5], d1[q], m[4] * d0[q]));
-              const float wnd = m[6] * d0[q] + m[7] * d1[q] + m[8] * d2;
-              const float gd = __builtin_fmaf(g0z[q], d2, gab[q]);
-              const float Q = __builtin_fmaf(-gd, wnd, __builtin_fmaf(w, d2, qab));
+              float Q = __builtin_fmaf(w, d2, qab);
+              if constexpr (!NOSUB) {
+                const float wnd = m[6] * d0[q] + m[7] * d1[q] + m[8] * d2;
+                const float gd = __builtin_fmaf(g0z[q], d2, gab[q]);
+                Q = __builtin_fmaf(-gd, wnd, Q);
+              }
               const float kern = Q * (ri2 * ri2 * ri);
               kab[q] += kern;
               v2[q] += d2 * kern;
@@ -506,9 +523,12 @@ __global__ __launch_bounds__(kBlock / T) void k_far(FarArgs A) {
               // d^T M d = M00 d0^2 + M11 d1^2 + S01 d0 d1 + d2 (S02 d0 + S12 d1 + M22 d2)
               const double qab = __builtin_fma(m[3], d01[q], __builtin_fma(m[1], d11[q], m[0] * d00[q]));
               const double w = __builtin_fma(m[2], d2, __builtin_fma(m[5], d1[q], m[4] * d0[q]));
-              const double wnd = m[6] * d0[q] + m[7] * d1[q] + m[8] * d2;
-              const double gd = __builtin_fma(g0z[q], d2, gab[q]);
-              const double Q = __builtin_fma(-gd, wnd, __builtin_fma(w, d2, qab));
+              double Q = __builtin_fma(w, d2, qab);
+              if constexpr (!NOSUB) {
+                const double wnd = m[6] * d0[q] + m[7] * d1[q] + m[8] * d2;
+                const double gd = __builtin_fma(g0z[q], d2, gab[q]);
+                Q = __builtin_fma(-gd, wnd, Q);
+              }
               const double kern = Q * ri5;
               kab[q] += kern;
               u2[q] += d2 * kern;
@@ -711,9 +731,31 @@ __global__ __launch_bounds__(kBlock) void k_far_sl_mfma(FarMfmaArgs A) {
 // Each target sums its slots in visiting order (one thread per target and
 // component), then v = scale*sum + xi*g0
 // combined into out as the direct finalize does.
+// Target data of coefficient column l (TreePlan::csub): g0 = -e_l for the far
+// field (tsub2) and for the leaf sums (tsub) at every target of the
+// on-surface plan and at the near targets of the off-surface plan.
+__global__ void k_csub_targets(int ntp, int l, int near_only, const int* tnear, double* tsub, double* tsub2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ntp) return;
+  for (int k = 0; k < 3; k++) {
+    const double v = k == l ? -1.0 : 0.0;
+    tsub2[(size_t)k * ntp + i] = v;
+    tsub[(size_t)k * ntp + i] = (!near_only || tnear[i]) ? v : 0.0;
+  }
+}
+
+// csub[(3 k + l) ntp + i] <- col[3 tperm[i] + k], column l of the
+// coefficients in sorted target order.
+__global__ void k_csub_store(int nt, int ntp, int l, const int* tperm, const double* col, double* csub) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nt) return;
+  const int o = tperm[i];
+  for (int k = 0; k < 3; k++) csub[(size_t)(3 * k + l) * ntp + i] = col[3 * (size_t)o + k];
+}
+
 __global__ void k_tree_finalize(int nt, int ntp, int nslot, const int* tperm, const int* slot_start,
                                 const double* slots, double scale, const double* xi, const double* g0,
-                                double* out, const double* base, double coef, int accumulate) {
+                                const double* csub, double* out, const double* base, double coef, int accumulate) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int k = blockIdx.y;   // component
   if (i >= nt) return;
@@ -722,6 +764,10 @@ __global__ void k_tree_finalize(int nt, int ntp, int nslot, const int* tperm, co
   double s = 0.0;
   for (int q = a; q < b; q++) s += slots[(size_t)k * nslot + q];
   double v = s * scale;
+  if (csub) {
+    const double* c = csub + (size_t)3 * k * ntp + i;
+    v -= (c[0] * g0[i] + c[ntp] * g0[ntp + i]) + c[2 * (size_t)ntp] * g0[2 * (size_t)ntp + i];
+  }
   if (xi) v += xi[i] * g0[(size_t)k * ntp + i];
   double w = coef * v;
   if (accumulate) out[3 * o + k] += w;
@@ -831,6 +877,8 @@ static int build_moment_tables(hor_plan* P, const double* sx) {
     HOR_TRY(TP.mom32.alloc((size_t)std::max(TP.nfar_nodes, 1) * (dl ? kNc32Dl : kNc32Sl) * TP.pflat));
   return HOR_OK;
 }
+
+static int tree_plan_csub(hor_plan* P);
 
 int tree_plan_create(hor_plan* P, const double* sx, const double* sn, const double* sa, const double* tx,
                      const double* tb, const double* tx0, const double* tn0, const double* ctr,
@@ -997,6 +1045,7 @@ int tree_plan_create(hor_plan* P, const double* sx, const double* sn, const doub
   HOR_TRY(build_moment_tables(P, sx));
   // xi of the treecode off-surface double layer (set by setup_targets for
   // method == HOR_TREE) uses the given g0 of every target.
+  if (dl && nt > 0) HOR_TRY(tree_plan_csub(P));
   return HOR_OK;
 }
 
@@ -1030,31 +1079,50 @@ int tree_plan_share_moments(hor_plan* owner, hor_plan* user, const double* sx) {
 }
 
 template <int KIND>
-static void launch_leaf(const LeafArgs& A, bool mixed, int nitems, hipStream_t s) {
+static void launch_leaf(const LeafArgs& A, bool mixed, bool nosub, int nitems, hipStream_t s) {
   const size_t lds = sizeof(double) * table_len(KIND);
-  if (mixed) hipLaunchKernelGGL((k_leaf<KIND, true>), dim3(nitems), dim3(kBlock), lds, s, A);
-  else hipLaunchKernelGGL((k_leaf<KIND, false>), dim3(nitems), dim3(kBlock), lds, s, A);
+  constexpr bool kDl = KIND == HOR_DL_ON || KIND == HOR_DL_OFF;
+  if (kDl && nosub) {
+    if (mixed) hipLaunchKernelGGL((k_leaf<KIND, true, kDl>), dim3(nitems), dim3(kBlock), lds, s, A);
+    else hipLaunchKernelGGL((k_leaf<KIND, false, kDl>), dim3(nitems), dim3(kBlock), lds, s, A);
+  } else {
+    if (mixed) hipLaunchKernelGGL((k_leaf<KIND, true, false>), dim3(nitems), dim3(kBlock), lds, s, A);
+    else hipLaunchKernelGGL((k_leaf<KIND, false, false>), dim3(nitems), dim3(kBlock), lds, s, A);
+  }
 }
 
-int tree_plan_eval(hor_plan* P, const double* dens_dev, const double* d0_dev, double* out_dev, const double* base,
-                   double coef, int accumulate) {
+// One evaluation of a treecode plan.  column < 0: the plan's operator on
+// dens (d0 the targets' given densities), written as base/coef/accumulate
+// say.  column l in 0..2: column l of the double layer's subtraction
+// coefficients (TreePlan::csub), FP64, into out (3 nt, the caller's order):
+// zero density, g0 = -e_l, no jump term.
+static int tree_eval(hor_plan* P, const double* dens_dev, const double* d0_dev, double* out_dev, const double* base,
+                     double coef, int accumulate, int column) {
   hor_ctx* c = P->ctx;
   hipStream_t s = c->stream;
   TreePlan& TP = *P->tree;
-  const bool mixed = P->precision == HOR_MIXED;
+  const bool coefs = column >= 0;
+  const bool mixed = P->precision == HOR_MIXED && !coefs;
   const bool dl = (P->kind == HOR_DL_ON || P->kind == HOR_DL_OFF);
+  const bool nosub = dl && TP.has_csub && !coefs;
   // density in tree order; per-target subtraction data (leaf and proxy variants)
   HOR_TRY(launch_gather3(dens_dev, P->sperm.p, P->ns, P->nsp, P->sx.p, P->sn.p, P->sa.p, P->sd.p, P->rec.p,
                          mixed ? P->rec32.p : nullptr, TP.leaf_box.p,
                          P->kind == HOR_SL_ON || P->kind == HOR_SL_OFF, s, TP.slot_sub.p));
-  HOR_TRY(prepare_targets(P, dens_dev, d0_dev));
-  if (P->kind == HOR_SL_OFF)
-    HOR_TRY(launch_target_f0n0_all(P->nt, P->ntp, P->tperm.p, d0_dev, P->n0_orig.p, P->tsub2.p, s));
-  else if (P->kind == HOR_DL_OFF)
-    HOR_TRY(launch_target_sub(4, P->nt, P->ntp, P->tperm.p, P->tnear.p, d0_dev, nullptr, nullptr, nullptr,
-                              P->tsub2.p, s));
-  else
-    HOR_HIP_TRY(hipMemcpyAsync(P->tsub2.p, P->tsub.p, sizeof(double) * 3 * P->ntp, hipMemcpyDeviceToDevice, s));
+  if (coefs) {
+    hipLaunchKernelGGL(k_csub_targets, dim3((P->ntp + kBlock - 1) / kBlock), dim3(kBlock), 0, s, P->ntp, column,
+                       P->kind == HOR_DL_OFF ? 1 : 0, P->tnear.p, P->tsub.p, P->tsub2.p);
+    HOR_HIP_TRY(hipGetLastError());
+  } else {
+    HOR_TRY(prepare_targets(P, dens_dev, d0_dev));
+    if (P->kind == HOR_SL_OFF)
+      HOR_TRY(launch_target_f0n0_all(P->nt, P->ntp, P->tperm.p, d0_dev, P->n0_orig.p, P->tsub2.p, s));
+    else if (P->kind == HOR_DL_OFF)
+      HOR_TRY(launch_target_sub(4, P->nt, P->ntp, P->tperm.p, P->tnear.p, d0_dev, nullptr, nullptr, nullptr,
+                                P->tsub2.p, s));
+    else
+      HOR_HIP_TRY(hipMemcpyAsync(P->tsub2.p, P->tsub.p, sizeof(double) * 3 * P->ntp, hipMemcpyDeviceToDevice, s));
+  }
   HOR_HIP_TRY(hipEventRecord(c->ev[2], s));
   HOR_HIP_TRY(hipEventRecord(TP.ev[0], s));
   // 1. moments (unless another plan computes them)
@@ -1096,10 +1164,10 @@ int tree_plan_eval(hor_plan* P, const double* dens_dev, const double* d0_dev, do
     A.item_src = TP.near_item_src.p;
     A.slots = TP.slots.p; A.nslot = TP.nslot;
     switch (P->kind) {
-      case HOR_SL_ON: launch_leaf<HOR_SL_ON>(A, mixed, TP.n_near_items, s); break;
-      case HOR_SL_OFF: launch_leaf<HOR_SL_OFF>(A, mixed, TP.n_near_items, s); break;
-      case HOR_DL_ON: launch_leaf<HOR_DL_ON>(A, mixed, TP.n_near_items, s); break;
-      default: launch_leaf<HOR_DL_OFF>(A, mixed, TP.n_near_items, s); break;
+      case HOR_SL_ON: launch_leaf<HOR_SL_ON>(A, mixed, false, TP.n_near_items, s); break;
+      case HOR_SL_OFF: launch_leaf<HOR_SL_OFF>(A, mixed, false, TP.n_near_items, s); break;
+      case HOR_DL_ON: launch_leaf<HOR_DL_ON>(A, mixed, nosub, TP.n_near_items, s); break;
+      default: launch_leaf<HOR_DL_OFF>(A, mixed, nosub, TP.n_near_items, s); break;
     }
     HOR_HIP_TRY(hipGetLastError());
   }
@@ -1128,12 +1196,15 @@ int tree_plan_eval(hor_plan* P, const double* dens_dev, const double* d0_dev, do
       G.tgt = F.tgt; G.slot = F.slot; G.far_idx = F.far_idx;
       G.tx = F.tx; G.tsub2 = F.tsub2; G.node_ctr = F.node_ctr; G.B = TP.bmat.p; G.slots = F.slots;
       hipLaunchKernelGGL(k_far_sl_mfma, grid, dim3(kBlock), 0, s, G);
+    } else if (dl && nosub) {
+      if (mixed) hipLaunchKernelGGL((k_far<true, true, kFarT32, true>), grid, b32, 0, s, F);
+      else hipLaunchKernelGGL((k_far<true, false, kFarT64, true>), grid, b64, 0, s, F);
     } else if (dl) {
-      if (mixed) hipLaunchKernelGGL((k_far<true, true, kFarT32>), grid, b32, 0, s, F);
-      else hipLaunchKernelGGL((k_far<true, false, kFarT64>), grid, b64, 0, s, F);
+      if (mixed) hipLaunchKernelGGL((k_far<true, true, kFarT32, false>), grid, b32, 0, s, F);
+      else hipLaunchKernelGGL((k_far<true, false, kFarT64, false>), grid, b64, 0, s, F);
     } else {
-      if (mixed) hipLaunchKernelGGL((k_far<false, true, kFarT32>), grid, b32, 0, s, F);
-      else hipLaunchKernelGGL((k_far<false, false, kFarT64>), grid, b64, 0, s, F);
+      if (mixed) hipLaunchKernelGGL((k_far<false, true, kFarT32, false>), grid, b32, 0, s, F);
+      else hipLaunchKernelGGL((k_far<false, false, kFarT64, false>), grid, b64, 0, s, F);
     }
     HOR_HIP_TRY(hipGetLastError());
   }
@@ -1144,8 +1215,39 @@ int tree_plan_eval(hor_plan* P, const double* dens_dev, const double* d0_dev, do
   hipLaunchKernelGGL(k_tree_finalize, dim3((P->nt + kBlock - 1) / kBlock, 3), dim3(kBlock), 0, s, P->nt, P->ntp,
                      TP.nslot, P->tperm.p, TP.slot_start.p, TP.slots.p,
                      dl ? -6.0 / (8.0 * kPi) : 1.0 / (8.0 * kPi),   // KITC.cpp:1395, 1612
-                     dl ? P->xi.p : nullptr, dl ? P->tsub2.p : nullptr, out_dev, base, coef, accumulate);
+                     dl && !coefs ? P->xi.p : nullptr, dl ? P->tsub2.p : nullptr, nosub ? TP.csub.p : nullptr,
+                     out_dev, base, coef, accumulate);
   HOR_HIP_TRY(hipGetLastError());
+  return HOR_OK;
+}
+
+int tree_plan_eval(hor_plan* P, const double* dens_dev, const double* d0_dev, double* out_dev, const double* base,
+                   double coef, int accumulate) {
+  return tree_eval(P, dens_dev, d0_dev, out_dev, base, coef, accumulate, -1);
+}
+
+// The double layer's subtraction coefficients (TreePlan::csub) of a new
+// treecode plan: three FP64 evaluations, before any moment sharing.
+static int tree_plan_csub(hor_plan* P) {
+  TreePlan& TP = *P->tree;
+  hipStream_t s = P->ctx->stream;
+  const int nt = P->nt;
+  DBuf<double> zero, col;
+  HOR_TRY(zero.alloc(3 * (size_t)std::max(P->ns, 1)));
+  HOR_TRY(col.alloc(3 * (size_t)nt));
+  HOR_TRY(TP.csub.alloc(9 * (size_t)P->ntp));
+  HOR_HIP_TRY(hipMemsetAsync(zero.p, 0, sizeof(double) * zero.n, s));
+  HOR_HIP_TRY(hipMemsetAsync(TP.csub.p, 0, sizeof(double) * TP.csub.n, s));
+  for (int l = 0; l < 3; l++) {
+    HOR_TRY(tree_eval(P, zero.p, nullptr, col.p, nullptr, 1.0, 0, l));
+    hipLaunchKernelGGL(k_csub_store, dim3((nt + kBlock - 1) / kBlock), dim3(kBlock), 0, s, nt, P->ntp, l,
+                       P->tperm.p, col.p, TP.csub.p);
+    HOR_HIP_TRY(hipGetLastError());
+  }
+  // the buffers above are freed on return: the stream must be done with them
+  HOR_HIP_TRY(hipStreamSynchronize(s));
+  TP.has_csub = true;
+  TP.evaluated = false;
   return HOR_OK;
 }
 

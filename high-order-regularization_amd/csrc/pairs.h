@@ -658,7 +658,10 @@ __device__ __forceinline__ void p_full(const SrcRecP& s, const Tgt& t, double dx
   u[0] += p_numer(s, t, dx, dy, dz) * H2;
 }
 
-template <int KIND>
+// Target data of sum kernel KIND.  NOSUB (double layer): the pair terms run on
+// g instead of g - g0, the subtraction being applied afterwards through the
+// plan's precomputed coefficients (tree plans, C_i g0_i in k_tree_finalize).
+template <int KIND, bool NOSUB = false>
 __device__ __forceinline__ void load_target(const DirectArgs& A, int i, Tgt& t) {
   int ntp = A.ntp;
   t.px = A.tx[i];
@@ -692,6 +695,7 @@ __device__ __forceinline__ void load_target(const DirectArgs& A, int i, Tgt& t) 
     t.n0x = A.tn0[i]; t.n0y = A.tn0[ntp + i]; t.n0z = A.tn0[2 * ntp + i];
     t.b = A.tb[i];
   }
+  if (NOSUB) t.g0x = t.g0y = t.g0z = 0.0;
 }
 
 // Sums over the sorted sources [j0, j1), which start at 64-subtile sub0.
