@@ -49,8 +49,15 @@ VALU_ISSUE = 256 * 4 * 16 * 2.4e9   # VALU lane-instructions/s: 256 CUs x 4 SIMD
 # its reduced form dx.w (w = -a f + (f0.n0) a n + (n0 x f0) x a n), fewer
 # operations than the reference's normal + tangential expression: plain 36,
 # regularized 63.
+# The treecode's double-layer plans apply the target-side subtraction g - g0
+# through per-target coefficients computed once per plan, so their matvec
+# kernels evaluate the pair and proxy expressions on g alone: the reference's
+# DL proxy interaction (KITC.cpp:748-771) without its 9 products and 9
+# differences g0_l W_n,m is 33 operations, and its on-surface and plain DL
+# pairs without g - g0 are 49 and 27.
 OPS = {"sl_plain": 30, "sl_reg": 61, "dl_plain": 30, "dl_reg": 127, "p_plain": 36, "p_reg": 63,
-       "dl_on_reg": 52, "dl_proxy": 51, "dl_moment": 37}
+       "dl_on_reg": 52, "dl_proxy": 51, "dl_moment": 37,
+       "dl_proxy_nosub": 33, "dl_on_reg_nosub": 49, "dl_plain_nosub": 27}
 PEAK_FP32 = 157.3e12         # MI355X FP32 (vector and matrix) FLOP/s
 
 
@@ -395,20 +402,22 @@ def onsurf_reg_fraction(S, pd, DEL, sample=2000, seed=3):
 
 def tree_roofline(S, phase, freg, mixed, counted=None):
     """Roofline figures of one treecode matvec's kernels from its phase times
-    (HIP events, ms) and the plans' work: the DL far field k_far (51 flops per
-    proxy interaction), the on-surface near sums k_leaf<DL_ON> (52 flops per
-    regularized pair, 30 per plain pair; the fraction regularized sampled) and
-    the moments (37 flops per moment term, FP64 matrix cores)."""
+    (HIP events, ms) and the plans' work, counted as the kernels evaluate it
+    (the subtraction g - g0 is applied through the plans' coefficients): the DL
+    far field k_far (33 flops per proxy interaction), the near sums k_leaf (49
+    flops per regularized on-surface pair, 27 per plain pair; the fraction
+    regularized sampled) and the moments (37 flops per moment term, FP64
+    matrix cores)."""
     st = [S.plan_stats(k) for k in range(4)]
     proxy = sum(x["proxy_pairs"] for x in st)
     near = sum(x["near_pairs"] for x in st)
     mom = sum(x["moment_terms"] for x in st)
     peak_far = PEAK_FP32 if mixed else PEAK_FP64
-    far_flops = proxy * OPS["dl_proxy"]
-    near_flops = near * (freg * OPS["dl_on_reg"] + (1 - freg) * OPS["dl_plain"])
+    far_flops = proxy * OPS["dl_proxy_nosub"]
+    near_flops = near * (freg * OPS["dl_on_reg_nosub"] + (1 - freg) * OPS["dl_plain_nosub"])
     mom_flops = mom * OPS["dl_moment"]
     out = {
-        "far": {"kernel": "k_far<true, true, 2>" if mixed else "k_far<true, false, 1>", "bound": "valu",
+        "far": {"kernel": "k_far<true, true, 2, true>" if mixed else "k_far<true, false, 1, true>", "bound": "valu",
                 "compute": "FP32 vector ALU, FP64 row sums" if mixed else "FP64 vector ALU",
                 "ms": phase["far_ms"], "flop": far_flops, "achieved": far_flops / (phase["far_ms"] * 1e-3) / 1e12,
                 "peak": peak_far / 1e12, "unit": "TFLOP/s",
@@ -424,7 +433,7 @@ def tree_roofline(S, phase, freg, mixed, counted=None):
         "work": {"proxy_interactions": proxy, "near_pairs": near, "moment_terms": mom},
     }
     if counted:
-        for key, name in (("far", out["far"]["kernel"]), ("near", "k_leaf<2, %s>" % ("true" if mixed else "false"))):
+        for key, name in (("far", out["far"]["kernel"]), ("near", "k_leaf<2, %s, true>" % ("true" if mixed else "false"))):
             c = counted.get(name)
             if c:
                 out[key]["counted"] = {k: c[k] for k in ("fp64_tflops", "fp64_frac_of_peak", "fp32_tflops",
