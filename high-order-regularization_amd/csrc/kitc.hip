@@ -85,8 +85,12 @@ struct TreePlan {
   // computed at plan creation by the plan's own FP64 kernels on zero density
   // with g0 = -e_l (the three columns); every later evaluation runs the
   // kernels on g alone (NOSUB) and k_tree_finalize subtracts csub g0.
+  // A plan computes them when asked (tree_plan_prepare_subtraction, which the
+  // solver calls for its plans before they share moments) or at its second
+  // evaluation, so a plan evaluated once does not pay for them.
   DBuf<double> csub;
   bool has_csub = false;
+  int n_evals = 0;
   // phase events of the last evaluation: before the moments, after the
   // moments, after the near sums, after the far field
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -878,8 +882,6 @@ static int build_moment_tables(hor_plan* P, const double* sx) {
   return HOR_OK;
 }
 
-static int tree_plan_csub(hor_plan* P);
-
 int tree_plan_create(hor_plan* P, const double* sx, const double* sn, const double* sa, const double* tx,
                      const double* tb, const double* tx0, const double* tn0, const double* ctr,
                      const hor_tree_params& tp) {
@@ -1045,7 +1047,6 @@ int tree_plan_create(hor_plan* P, const double* sx, const double* sn, const doub
   HOR_TRY(build_moment_tables(P, sx));
   // xi of the treecode off-surface double layer (set by setup_targets for
   // method == HOR_TREE) uses the given g0 of every target.
-  if (dl && nt > 0) HOR_TRY(tree_plan_csub(P));
   return HOR_OK;
 }
 
@@ -1221,15 +1222,33 @@ static int tree_eval(hor_plan* P, const double* dens_dev, const double* d0_dev, 
   return HOR_OK;
 }
 
+static int tree_plan_csub(hor_plan* P);
+
 int tree_plan_eval(hor_plan* P, const double* dens_dev, const double* d0_dev, double* out_dev, const double* base,
                    double coef, int accumulate) {
+  TreePlan& TP = *P->tree;
+  const bool dl = P->kind == HOR_DL_ON || P->kind == HOR_DL_OFF;
+  if (dl && !TP.has_csub && !TP.mom_owner && ++TP.n_evals == 2) HOR_TRY(tree_plan_csub(P));
   return tree_eval(P, dens_dev, d0_dev, out_dev, base, coef, accumulate, -1);
 }
 
-// The double layer's subtraction coefficients (TreePlan::csub) of a new
-// treecode plan: three FP64 evaluations, before any moment sharing.
+int tree_plan_prepare_subtraction(hor_plan* P) {
+  if (!P || !P->tree) return set_error(HOR_E_ARG, "subtraction coefficients: not a treecode plan");
+  if (P->kind != HOR_DL_ON && P->kind != HOR_DL_OFF) return HOR_OK;
+  if (P->tree->has_csub) return HOR_OK;
+  if (P->tree->mom_owner)
+    return set_error(HOR_E_STATE, "subtraction coefficients: the plan reads another plan's moments");
+  return tree_plan_csub(P);
+}
+
+// The double layer's subtraction coefficients (TreePlan::csub) of a treecode
+// plan that computes its own moments: three FP64 evaluations.
 static int tree_plan_csub(hor_plan* P) {
   TreePlan& TP = *P->tree;
+  if (P->nt == 0) {
+    TP.has_csub = true;
+    return HOR_OK;
+  }
   hipStream_t s = P->ctx->stream;
   const int nt = P->nt;
   DBuf<double> zero, col;

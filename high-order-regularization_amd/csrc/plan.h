@@ -65,6 +65,9 @@ int tree_plan_create(hor_plan* P, const double* sx, const double* sn, const doub
 // The user plan reads the moments the owner plan computes for both (same
 // source tree and density; the owner is evaluated first).
 int tree_plan_share_moments(hor_plan* owner, hor_plan* user, const double* sx);
+// The double layer's subtraction coefficients of a treecode plan (a no-op for
+// the single layer), before it reads another plan's moments.
+int tree_plan_prepare_subtraction(hor_plan* P);
 int tree_plan_eval(hor_plan* P, const double* dens_dev, const double* d0_dev, double* out_dev,
                    const double* base, double coef, int accumulate);
 // HOR_FAR_VALU or HOR_FAR_MFMA (mixed-precision single-layer plans only).

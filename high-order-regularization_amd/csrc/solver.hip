@@ -168,7 +168,11 @@ static int ensure_fp64_plans(hor_solver* S) {
     HOR_TRY(hor_plan_create(c, HOR_DL_OFF, S->method, HOR_FP64, S->DEL, nk, S->x[k].data(), S->nrml[k].data(),
                             S->area[k].data(), no, S->x[o].data(), S->b[o].data(), S->x0[o].data(),
                             S->n0[o].data(), S->ctr[k], &S->tp[k], &S->off64[k]));
-    if (S->method == HOR_TREE) HOR_TRY(tree_plan_share_moments(S->on64[k], S->off64[k], S->x[k].data()));
+    if (S->method == HOR_TREE) {
+      HOR_TRY(tree_plan_prepare_subtraction(S->on64[k]));
+      HOR_TRY(tree_plan_prepare_subtraction(S->off64[k]));
+      HOR_TRY(tree_plan_share_moments(S->on64[k], S->off64[k], S->x[k].data()));
+    }
   }
   return HOR_OK;
 }
@@ -228,7 +232,11 @@ extern "C" int hor_solver_create(hor_ctx* ctx, double h, double DEL, int method,
     HOR_TRY(hor_plan_create(ctx, HOR_DL_OFF, method, precision, DEL, me->n, me->x, me->nrml, me->area,
                             other->n, other->x, other->b, other->x0, other->n0, me->ctr, &S->tp[k],
                             &S->off[k]));
-    if (method == HOR_TREE) HOR_TRY(tree_plan_share_moments(S->on[k], S->off[k], me->x));
+    if (method == HOR_TREE) {
+      HOR_TRY(tree_plan_prepare_subtraction(S->on[k]));
+      HOR_TRY(tree_plan_prepare_subtraction(S->off[k]));
+      HOR_TRY(tree_plan_share_moments(S->on[k], S->off[k], me->x));
+    }
     // stencils of the points of surface k (their nearest points lie on the other surface)
     HOR_TRY(S->st[k].alloc(36 * (size_t)me->n));
     HOR_TRY(S->co[k].alloc(12 * (size_t)me->n));

@@ -55,6 +55,41 @@ def test_tree_reference_defaults_spheroid_pair(ctx, oracle):
     _check(hor.ops.eval_Stokes_DL_onSurf_tree(ctx, b1.ctr, h, DEL, s1), oracle.dl_on_tree(tp_o, DEL, s1))
 
 
+def test_tree_dl_reused_plan_subtraction_coefficients(ctx, oracle, ell):
+    """A double-layer treecode plan evaluated again and again: the first
+    evaluation subtracts g0 inside its kernels, the second computes the plan's
+    subtraction coefficients C_i and sums on g alone (D[g] - C g0), and so do
+    the later ones.  Every evaluation, on new seeded densities, against the
+    oracle's treecode.  The off-surface targets include 20 placed on sources
+    (b = 0, a self pair in the leaf sums) and 20 at delta/2 inside the surface
+    below a source (x0 = y, b = -delta/2)."""
+    body, s, t, DEL = ell
+    ctr = body.ctr
+    P8 = hor.TreeParams(8, 300, 0.6)
+    tp_o = oracle.tree_params(8, 300, 0.6, ctr)
+    nt = t["n"]
+    t = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in t.items()}
+    y, n = s["x"][:120].reshape(-1, 3), s["nrml"][:120].reshape(-1, 3)
+    b = np.where(np.arange(40) < 20, 0.0, -0.5 * DEL)
+    t["x"][:120] = (y + b[:, None] * n).ravel()
+    t["x0"][:120] = y.ravel()
+    t["n0"][:120] = n.ravel()
+    t["b"][:40] = b
+    rng = np.random.default_rng(23)
+    p_on = hor.Plan(ctx, hor.DL_ON, s, None, DEL, method=hor.TREE, ctr=ctr, tree=P8)
+    p_off = hor.Plan(ctx, hor.DL_OFF, s, t, DEL, method=hor.TREE, ctr=ctr, tree=P8)
+    for it in range(3):
+        s["g"] = rng.uniform(-1, 1, 3 * s["n"])
+        t["g0"] = rng.uniform(-1, 1, 3 * nt)
+        on, off = p_on(s["g"]), p_off(s["g"], t["g0"])
+        e_on, e_off = rel_l2(on, oracle.dl_on_tree(tp_o, DEL, s)), rel_l2(off, oracle.dl_off_tree(tp_o, DEL, s, t))
+        print(f"evaluation {it + 1}: DL_ON {e_on:.2e}, DL_OFF {e_off:.2e}", flush=True)
+        _check(on, oracle.dl_on_tree(tp_o, DEL, s))
+        _check(off, oracle.dl_off_tree(tp_o, DEL, s, t))
+    p_on.close()
+    p_off.close()
+
+
 def test_tree_single_leaf_equals_direct(ctx, ell):
     """With N0 > N the root is one leaf: the treecode is the direct sum."""
     body, s, t, DEL = ell
