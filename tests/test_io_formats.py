@@ -4,11 +4,17 @@ velocity dump layout (highreg_stokes_2surf/main.cpp:258-270) and the two
 mid-plane slice layouts (extend_to_grid/main.cpp:294-303, 543-549)."""
 import ctypes as C
 import ctypes.util
+import os
+import subprocess
 
 import numpy as np
+import pytest
 
+from conftest import ROOT
 from hor import grid as HG
 from hor import io as HIO
+
+REF_MAIN = os.path.join(ROOT, "oracle", "_ref", "ref2surf_main")
 
 
 def _c_fmt(v):
@@ -34,6 +40,40 @@ def test_velocity_dump_round_trip(tmp_path):
     assert len(lines) == 50 and all(len(line.split(" ")) == 3 for line in lines)
     back = HIO.read_velocity(p)
     assert np.abs(back - u).max() <= 1e-15 * np.abs(u).max()
+
+
+def test_velocity_dump_bytes_match_the_reference_program(tmp_path):
+    """The reference two-surface program (its own sources, built under
+    oracle/_ref) writes its velocities with setprecision(16)
+    (main.cpp:258-270).  read_velocity then write_velocity on those dumps
+    gives back the same bytes, except for numbers whose 16-digit decimal does
+    not name its double (printing the parsed double gives another string):
+    those come back as %.16g of that double, as the reference would print it."""
+    if not os.path.exists(REF_MAIN):
+        pytest.skip("reference program (oracle/_ref/ref2surf_main) not built")
+    run = subprocess.run([REF_MAIN], cwd=tmp_path, capture_output=True, text=True, timeout=600)
+    assert run.returncode == 0, run.stderr[-2000:]
+    for name in ("vel1_N32_highreg_57_k4", "vel2_N32_highreg_57_k4"):
+        ref = (tmp_path / name).read_text()
+        HIO.write_velocity(tmp_path / "again", HIO.read_velocity(tmp_path / name))
+        ours = (tmp_path / "again").read_text()
+        lines = ref.split("\n")
+        assert lines[-1] == "" and all(len(line.split(" ")) == 3 for line in lines[:-1])
+        renamed = 0
+        expect = []
+        for line in lines[:-1]:
+            toks = []
+            for tok in line.split(" "):
+                again = "%.16g" % float(tok)
+                renamed += again != tok
+                toks.append(again)
+            expect.append(" ".join(toks))
+        print(f"{name}: {len(lines) - 1} lines, {renamed} of {3 * (len(lines) - 1)} numbers not named by "
+              f"their 16 digits", flush=True)
+        assert renamed <= 0.05 * 3 * (len(lines) - 1)
+        assert ours == "\n".join(expect) + "\n"
+        same = sum(a == b for a, b in zip(ours.split("\n"), lines))
+        assert same >= 0.9 * len(lines)
 
 
 def test_mid_plane_slice_layouts(tmp_path):
