@@ -57,6 +57,12 @@ __device__ __forceinline__ double dl_on_G(const double* __restrict__ tab, double
 }
 
 
+// The active lanes of the wave where the predicate holds (a scalar mask).  The
+// per-source votes take the ballot of the distance compare alone and AND it
+// with the loop-invariant mask of near targets, so a vote is one vector
+// compare and scalar instructions.
+__device__ __forceinline__ unsigned long long wave_mask(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+
 // Box distance test between a wave's target box and a source subtile box.
 __device__ __forceinline__ bool boxes_far(const double* wb, const double* sb, double far_d2) {
   double d2 = 0.0;
@@ -724,6 +730,7 @@ __device__ __forceinline__ void range_sum(const DirectArgs& A, int j0, int j1, i
   const double self1 = 32.0 / 5.0 / rp / A.DEL;
   const double self2 = 256.0 / 15.0 / rp / (A.DEL * A.DEL * A.DEL);
   const SrcRec* __restrict__ R = A.rec;
+  const unsigned long long nearm = wave_mask(t.nearp);   // lanes whose target is near its surface
   const SrcRec32* __restrict__ R32 = A.rec32;
   constexpr bool SL = (KIND == HOR_SL_ON || KIND == HOR_SL_OFF);
   const double k5 = vreg(4.375);   // rsqrt5_64's e^2 coefficient, in vector registers for the loops
@@ -765,15 +772,14 @@ __device__ __forceinline__ void range_sum(const DirectArgs& A, int j0, int j1, i
         const SrcRec s = R[j];
         const double dx = t.px - s.x, dy = t.py - s.y, dz = t.pz - s.z;
         const double r2 = dx * dx + dy * dy + dz * dz;
-        const bool need = t.nearp && !(r2 > A.near_r2);
-        if (MIXED && KIND != HOR_DL_ON && !__any((t.nearp && !(r2 > A.near64_r2)) ? 1 : 0) &&
-            __any(need ? 1 : 0)) {
+        const unsigned long long need = wave_mask(!(r2 > A.near_r2)) & nearm;
+        if (MIXED && KIND != HOR_DL_ON && need && !(wave_mask(!(r2 > A.near64_r2)) & nearm)) {
           const SrcRec32& s32 = R32[j];
           const float fx = (float)dx, fy = (float)dy, fz = (float)dz;
           if (SL) sl_reg32(s32, fx, fy, fz, v, t.nearp, inv_del32, w);
           else if (KIND == HOR_DL_OFF) dl_off_reg32(s32, fx, fy, fz, v, t.nearp, inv_del32, w);
           else dl_on_reg32(s32, fx, fy, fz, v, inv_del32, w);
-        } else if (__any(need ? 1 : 0)) {
+        } else if (need) {
           if (SL) sl_full(s, t.px, t.py, t.pz, t.sub, t.nearp, t.q, inv_del, inv_del2, self1, self2, tab, u);
           else if (KIND == HOR_DL_OFF) dl_off_full(s, t, inv_del, inv_del2, self2, tab, u);
           else dl_on_full(s, t, inv_del2, inv_del5, tab, k5, u);
@@ -803,6 +809,7 @@ __device__ __forceinline__ void range_sum_p(const DirectArgs& A, int j0, int j1,
   const double inv_del2 = inv_del * inv_del;
   const double self2 = 256.0 / 15.0 / rp / (A.DEL * A.DEL * A.DEL);
   const SrcRecP* __restrict__ R = A.recp;
+  const unsigned long long nearm = wave_mask(t.nearp);   // lanes whose target is near its surface
   const double k3 = vreg(1.875);   // rsqrt3_64's e^2 coefficient, in vector registers for the loops
   for (int base = j0, sb = sub0; base < j1; base += kWave, sb++) {
     const int end = min(j1, base + kWave);
@@ -817,8 +824,7 @@ __device__ __forceinline__ void range_sum_p(const DirectArgs& A, int j0, int j1,
         const SrcRecP s = R[j];
         const double dx = v.px - s.x, dy = v.py - s.y, dz = v.pz - s.z;
         const double r2 = dx * dx + dy * dy + dz * dz;
-        const bool need = t.nearp && !(r2 > A.near_r2);
-        if (__any(need ? 1 : 0)) p_full(s, t, dx, dy, dz, r2, inv_del, inv_del2, self2, tab, u);
+        if (wave_mask(!(r2 > A.near_r2)) & nearm) p_full(s, t, dx, dy, dz, r2, inv_del, inv_del2, self2, tab, u);
         else p_plain_r2(s, t, v, r2, k3, u);
       }
     }
@@ -841,6 +847,7 @@ __device__ __forceinline__ void range_sum_sldl(const DirectArgs& A, int j0, int 
   const double self1 = 32.0 / 5.0 / rp / A.DEL;
   const double self2 = 256.0 / 15.0 / rp / (A.DEL * A.DEL * A.DEL);
   const SrcRecV* __restrict__ R = A.recv;
+  const unsigned long long nearm = wave_mask(t.nearp);   // lanes whose target is near its surface
   const SrcRecV32* __restrict__ R32 = A.recv32;
   Tgt32 v;
   if (MIXED) make_tgt32(t, v);
@@ -874,10 +881,10 @@ __device__ __forceinline__ void range_sum_sldl(const DirectArgs& A, int j0, int 
         const SrcRecV s = R[j];
         const double dx = t.px - s.x, dy = t.py - s.y, dz = t.pz - s.z;
         const double r2 = dx * dx + dy * dy + dz * dz;
-        const bool need = t.nearp && !(r2 > A.near_r2);
-        if (MIXED && !__any((t.nearp && !(r2 > A.near64_r2)) ? 1 : 0) && __any(need ? 1 : 0))
+        const unsigned long long need = wave_mask(!(r2 > A.near_r2)) & nearm;
+        if (MIXED && need && !(wave_mask(!(r2 > A.near64_r2)) & nearm))
           sldl_reg32(R32[j], (float)dx, (float)dy, (float)dz, v, t.nearp, inv_del32, ws, wd);
-        else if (__any(need ? 1 : 0)) sldl_full(s, t, inv_del, inv_del2, self1, self2, tab, us, ud);
+        else if (need) sldl_full(s, t, inv_del, inv_del2, self1, self2, tab, us, ud);
         else sldl_plain(s, t, us, ud);
       }
       if (MIXED) {
