@@ -1,7 +1,7 @@
 """Writes the golden fixtures of tests/golden from the compiled reference.
 
 Runs only where the reference sources were built into oracle/_ref (this
-needs /root/reference at build time); the GPU box only reads the .npz files.
+needs /root/reference at build time); the GPU tests only read the .npz files.
 Inputs come from the reference's own geometry routines (Generate_Surface,
 Compute_nearest_data, set_surface_force, find_nrst_surfc) and seeded
 densities; outputs are the reference functions' results.
