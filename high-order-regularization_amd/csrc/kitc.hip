@@ -15,8 +15,8 @@
 //                  targets (same pair code and LDS tiling as the direct kernel);
 //   3. k_far:      plain Stokeslet / stresslet from the (P+1)^3 proxies with
 //                  target-side subtraction (Call_Treecode_SL/DL);
-//   4. k_tree_finalize: each target adds its list's slots in visiting order,
-//                  scales and adds the jump term.
+//   4. k_tree_finalize: one wavefront per target adds its slots (a fixed-order
+//                  reduction), scales, subtracts and adds the jump term.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -732,9 +732,6 @@ __global__ __launch_bounds__(kBlock) void k_far_sl_mfma(FarMfmaArgs A) {
   }
 }
 
-// Each target sums its slots in visiting order (one thread per target and
-// component), then v = scale*sum + xi*g0
-// combined into out as the direct finalize does.
 // Target data of coefficient column l (TreePlan::csub): g0 = -e_l for the far
 // field (tsub2) and for the leaf sums (tsub) at every target of the
 // on-surface plan and at the near targets of the off-surface plan.
@@ -757,17 +754,37 @@ __global__ void k_csub_store(int nt, int ntp, int l, const int* tperm, const dou
   for (int k = 0; k < 3; k++) csub[(size_t)(3 * k + l) * ntp + i] = col[3 * (size_t)o + k];
 }
 
-__global__ void k_tree_finalize(int nt, int ntp, int nslot, const int* tperm, const int* slot_start,
-                                const double* slots, double scale, const double* xi, const double* g0,
-                                const double* csub, double* out, const double* base, double coef, int accumulate) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int k = blockIdx.y;   // component
-  if (i >= nt) return;
-  int o = tperm[i];
-  int a = slot_start[i], b = slot_start[i + 1];   // sorted order: a wave reads neighbouring slots
-  double s = 0.0;
-  for (int q = a; q < b; q++) s += slots[(size_t)k * nslot + q];
-  double v = s * scale;
+// Per-target reduction of the slots (one wavefront per target): lane l adds
+// the target's slots a + l, a + l + 64, ... of each component (coalesced
+// reads of the target's contiguous slot range), a butterfly adds the 64
+// partial sums (every lane ends with the same total, in the same order on
+// every run), and lane k < 3 finishes component k: v = scale*sum, minus the
+// subtraction coefficients' C g0 (NOSUB), plus the jump term xi*g0, combined
+// into out as the direct finalize does.
+__global__ __launch_bounds__(kBlock) void k_tree_finalize(int nt, int ntp, int nslot, const int* tperm,
+                                                          const int* slot_start, const double* slots, double scale,
+                                                          const double* xi, const double* g0, const double* csub,
+                                                          double* out, const double* base, double coef,
+                                                          int accumulate) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int i = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+  if (i >= nt) return;   // the whole wave
+  const int a = slot_start[i], b = slot_start[i + 1];
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  for (int q = a + lane; q < b; q += kWave) {
+    s0 += slots[q];
+    s1 += slots[(size_t)nslot + q];
+    s2 += slots[2 * (size_t)nslot + q];
+  }
+  for (int m = kWave / 2; m > 0; m >>= 1) {
+    s0 += __shfl_xor(s0, m);
+    s1 += __shfl_xor(s1, m);
+    s2 += __shfl_xor(s2, m);
+  }
+  if (lane >= 3) return;
+  const int k = lane;   // component
+  const int o = tperm[i];
+  double v = (k == 0 ? s0 : (k == 1 ? s1 : s2)) * scale;
   if (csub) {
     const double* c = csub + (size_t)3 * k * ntp + i;
     v -= (c[0] * g0[i] + c[ntp] * g0[ntp + i]) + c[2 * (size_t)ntp] * g0[2 * (size_t)ntp + i];
@@ -952,7 +969,7 @@ int tree_plan_create(hor_plan* P, const double* sx, const double* sn, const doub
   // surface's points are) the leaves' sources are split into chunks of whole
   // 64-source subtiles so that the near sum still has about kNearWantItems
   // workgroups; each chunk of a (target, leaf) entry gets a slot of its own,
-  // consecutive, summed in order by k_tree_finalize.
+  // consecutive.
   const int split = std::max(1, std::min(kNearMaxSplit, (kNearWantItems + std::max(1, (int)ni_node.size()) - 1) /
                                                             std::max(1, (int)ni_node.size())));
   std::vector<int> nchunk(nn, 1), csub(nn, 1);
@@ -963,7 +980,7 @@ int tree_plan_create(hor_plan* P, const double* sx, const double* sn, const doub
     nchunk[q] = (nsub + csub[q] - 1) / csub[q];
   }
   // slots of each list entry, targets in sorted order (the finalize kernel's
-  // threads), a target's entries in visiting order
+  // wavefronts), a target's entries contiguous in visiting order
   std::vector<int> new_start(TP.L.node.size(), 0), slot_start(nt + 1, 0);
   for (int i = 0, next = 0; i < nt; i++) {
     const int o = P->tperm_h[i];
@@ -1213,7 +1230,8 @@ static int tree_eval(hor_plan* P, const double* dens_dev, const double* d0_dev, 
   HOR_HIP_TRY(hipEventRecord(TP.ev[3], s));
   TP.evaluated = true;
   // 4. per-target reduction
-  hipLaunchKernelGGL(k_tree_finalize, dim3((P->nt + kBlock - 1) / kBlock, 3), dim3(kBlock), 0, s, P->nt, P->ntp,
+  hipLaunchKernelGGL(k_tree_finalize, dim3((P->nt + kBlock / kWave - 1) / (kBlock / kWave)), dim3(kBlock), 0, s,
+                     P->nt, P->ntp,
                      TP.nslot, P->tperm.p, TP.slot_start.p, TP.slots.p,
                      dl ? -6.0 / (8.0 * kPi) : 1.0 / (8.0 * kPi),   // KITC.cpp:1395, 1612
                      dl && !coefs ? P->xi.p : nullptr, dl ? P->tsub2.p : nullptr, nosub ? TP.csub.p : nullptr,
