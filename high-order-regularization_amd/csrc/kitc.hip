@@ -158,27 +158,33 @@ typedef double f64x4_t __attribute__((ext_vector_type(4)));
 // independent, so consecutive MFMAs do not wait on each other.
 constexpr int kMomTiles = 8;   // np1*ncomp <= 13*9 = 117 columns (P <= 12)
 
-// One step of four members for the NT column tiles (IN: some members of the
-// step are past the chunk and contribute zero).
-template <int NT, bool IN>
+// One step of four members for the NT column tiles (operands in LDS).
+template <int NT>
 __device__ __forceinline__ void moment_step(const double* fm, const double* qm, int aj, int ak, int dj,
-                                            const int* bi, const int* bc, bool in, f64x4_t* acc) {
-  const double a = (!IN || in) ? fm[aj] * fm[ak] : 0.0;
-  const double d = (!IN || in) ? fm[dj] : 0.0;
+                                            const int* bi, const int* bc, f64x4_t* acc) {
+  const double a = fm[aj] * fm[ak];
+  const double d = fm[dj];
 #pragma unroll
   for (int t = 0; t < NT; t++) {
-    const double b = (!IN || in) ? fm[bi[t]] * d * qm[bc[t]] : 0.0;
+    const double b = fm[bi[t]] * d * qm[bc[t]];
     acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[t], 0, 0, 0);
   }
 }
 
+// Members staged in LDS per round: their barycentric-factor rows (up to
+// 3 * 13 + 1 values for P <= 12) and A q rows (16 values), loaded coalesced
+// by the whole workgroup; the four waves read their operands from LDS.
+constexpr int kMomStage = 32, kMomFacMax = 3 * 13 + 1;
+
 template <int NT>
 __global__ __launch_bounds__(kBlock) void k_moments(MomArgs A) {
+  __shared__ double sfac[kMomStage * kMomFacMax];
+  __shared__ double sq[kMomStage * 16];
   const int item = blockIdx.x;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int np1 = A.np1, ncomp = A.ncomp, nrow = np1 * np1, ncol = np1 * ncomp;
   const int row0 = (blockIdx.y * 4 + wave) * 16;
-  if (row0 >= nrow) return;  // wave-uniform
+  const bool rows_ok = row0 < nrow;  // wave-uniform; every wave stages members and meets the barriers
   const int f = A.item_far[item];
   const int node = A.far_node[f];
   const int lo = A.node_lo[node];
@@ -199,16 +205,24 @@ __global__ __launch_bounds__(kBlock) void k_moments(MomArgs A) {
   f64x4_t acc[NT];
 #pragma unroll
   for (int t = 0; t < NT; t++) acc[t] = f64x4_t{0.0, 0.0, 0.0, 0.0};
-  const double* fm = A.fac + A.fac_off[f] + (size_t)(c0 + ms) * stride;
-  const double* qm = A.mq + (size_t)(lo + c0 + ms) * 16;
+  const double* fsrc = A.fac + A.fac_off[f] + (size_t)c0 * stride;
+  const double* qsrc = A.mq + (size_t)(lo + c0) * 16;
   const int n = c1 - c0;
-  const int nmain = n & ~3;  // whole steps of four members: no bounds tests
-  for (int m0 = 0; m0 < nmain; m0 += 4) {
-    moment_step<NT, false>(fm, qm, aj, ak, dj, bi, bc, true, acc);
-    fm += (size_t)4 * stride;
-    qm += 64;
+  for (int m = 0; m < n; m += kMomStage) {
+    const int nm = min(kMomStage, n - m);
+    __syncthreads();  // the previous round's operands have been read
+    // members past the chunk are zero rows: their products add nothing
+    for (int e = threadIdx.x; e < kMomStage * stride; e += kBlock)
+      sfac[e] = e < nm * stride ? fsrc[(size_t)m * stride + e] : 0.0;
+    for (int e = threadIdx.x; e < kMomStage * 16; e += kBlock) sq[e] = e < nm * 16 ? qsrc[(size_t)m * 16 + e] : 0.0;
+    __syncthreads();
+    if (rows_ok) {
+      const int steps = (nm + 3) / 4;
+      for (int st = 0; st < steps; st++)
+        moment_step<NT>(sfac + (st * 4 + ms) * stride, sq + (st * 4 + ms) * 16, aj, ak, dj, bi, bc, acc);
+    }
   }
-  if (nmain < n) moment_step<NT, true>(fm, qm, aj, ak, dj, bi, bc, nmain + ms < n, acc);
+  if (!rows_ok) return;
   double* P = A.part + (size_t)item * A.pflat * ncomp;
 #pragma unroll
   for (int t = 0; t < NT; t++) {
