@@ -26,7 +26,10 @@ def ell():
     return ellipsoid_case(h=1 / 16, max_band=2000)
 
 
-@pytest.mark.parametrize("P,N0,theta", [(6, 200, 0.6), (8, 300, 0.6), (4, 150, 0.8)])
+# P = 12 is the largest order a plan accepts: 13 x 13 proxy rows (the moment
+# GEMM's third row block has a wave past the last row, which only stages
+# members) and 117 weight columns (eight column tiles).
+@pytest.mark.parametrize("P,N0,theta", [(6, 200, 0.6), (8, 300, 0.6), (4, 150, 0.8), (12, 300, 0.6)])
 def test_tree_four_ops(ctx, oracle, ell, P, N0, theta):
     body, s, t, DEL = ell
     ctr = body.ctr
