@@ -44,10 +44,11 @@ __device__ __forceinline__ void erf_exp_AE(const double* __restrict__ tab, doubl
 }
 
 // G(s) = F(rho)/rho^5 of the on-surface kernel from its piecewise polynomial
-// (dl_on_table.h), tab in LDS: interval k = floor(s / w), local x in [-1, 1].
-__device__ __forceinline__ double dl_on_G(const double* __restrict__ tab, double s) {
-  const int k = min((int)(s * (kDlOnIntervals / kDlOnSmax)), kDlOnIntervals - 1);
-  const double x = s * (2.0 * kDlOnIntervals / kDlOnSmax) - (double)(2 * k + 1);
+// (dl_on_table.h), tab in LDS, at v = s / w (the interval coordinate, w the
+// interval width): interval k = floor(v), local x = 2 v - (2 k + 1) in [-1, 1].
+__device__ __forceinline__ double dl_on_G(const double* __restrict__ tab, double v) {
+  const int k = min((int)v, kDlOnIntervals - 1);
+  const double x = 2.0 * v - (double)(2 * k + 1);
   const double* c = tab + k;
   constexpr int S = kDlOnIntervals;
   double g = c[kDlOnDeg * S];
@@ -348,17 +349,19 @@ __device__ __forceinline__ void sldl_full(const SrcRecV& s, const Tgt& t, double
 // H3 = F(rho)/r^5 with F = erf(rho) + p3(rho^2) rho e^{-rho^2}; within 8 delta
 // H3 = G(rho^2)/delta^5 from the table (no exp, erf or square root), beyond
 // it the plain 1/r^5.
-__device__ __forceinline__ void dl_on_full(const SrcRec& s, const Tgt& t, double inv_del2, double inv_del5,
+// vsc = (1/delta^2) / w, w = kDlOnSmax / kDlOnIntervals (a power of two), so
+// v = r^2 vsc is (r/delta)^2 / w exactly and the 8 delta test is v > 64 / w.
+__device__ __forceinline__ void dl_on_full(const SrcRec& s, const Tgt& t, double vsc, double inv_del5,
                                            const double* __restrict__ tab, double k5, double u[3]) {
   double dx = t.px - s.x, dy = t.py - s.y, dz = t.pz - s.z;
   double r2 = dx * dx + dy * dy + dz * dz;
   if (r2 < 1e-28) return;  // d = 0: the self term contributes exactly zero
-  const double sq = r2 * inv_del2;
+  const double v = r2 * vsc;
   double H3;
-  if (sq > 64.0) {
+  if (v > 64.0 * (kDlOnIntervals / kDlOnSmax)) {
     H3 = rsqrt5_64(r2, k5);
   } else {
-    H3 = dl_on_G(tab, sq) * inv_del5;
+    H3 = dl_on_G(tab, v) * inv_del5;
   }
   double gx = s.d0 - t.g0x, gy = s.d1 - t.g0y, gz = s.d2 - t.g0z;
   double dxn = dx * s.nx + dy * s.ny + dz * s.nz;
@@ -782,7 +785,7 @@ __device__ __forceinline__ void range_sum(const DirectArgs& A, int j0, int j1, i
         } else if (need) {
           if (SL) sl_full(s, t.px, t.py, t.pz, t.sub, t.nearp, t.q, inv_del, inv_del2, self1, self2, tab, u);
           else if (KIND == HOR_DL_OFF) dl_off_full(s, t, inv_del, inv_del2, self2, tab, u);
-          else dl_on_full(s, t, inv_del2, inv_del5, tab, k5, u);
+          else dl_on_full(s, t, inv_del2 * (kDlOnIntervals / kDlOnSmax), inv_del5, tab, k5, u);
         } else {
           if (SL) sl_plain(s, t.px, t.py, t.pz, t.sub, u);
           else dl_plain(s, t.px, t.py, t.pz, t.g0x, t.g0y, t.g0z, k5, u);
