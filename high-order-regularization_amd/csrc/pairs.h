@@ -575,20 +575,12 @@ __device__ __forceinline__ void dl_on_reg32(const SrcRec32& s, float dx, float d
 // ---- Pressure pair (stokes_pressure_7ord, extend_to_grid/stokes.cpp:134-203).
 // The reference's summand [-(f.n - f0.n0)(n.dx) - (n x f - n0 x f0).(n x dx)] A
 // equals dx.w with w = -(a f) + (f0.n0)(a n) + (n0 x f0) x (a n) for a unit
-// normal n (Binet-Cauchy and the triple product).  Tgt.sub holds f0.n0 and
-// Tgt.g0 holds n0 x f0.  dx.w is evaluated as
-//   (f0.n0)(a n . dx) - a f . dx + a n . (dx x (n0 x f0)):
-// every product then reads at most one source value (the wave holds the
-// record in scalar registers and a VALU instruction reads at most one), so
-// no record value is first copied to vector registers.
-template <class Rec>
-__device__ __forceinline__ double p_numer(const Rec& s, const Tgt& t, double dx, double dy, double dz) {
-  const double nd = s.nx * dx + s.ny * dy + s.nz * dz;
-  const double fd = s.d0 * dx + s.d1 * dy + s.d2 * dz;
-  const double cx = dy * t.g0z - dz * t.g0y, cy = dz * t.g0x - dx * t.g0z, cz = dx * t.g0y - dy * t.g0x;
-  const double nc = s.nx * cx + s.ny * cy + s.nz * cz;
-  return (t.sub * nd - fd) + nc;
-}
+// normal n (Binet-Cauchy and the triple product), i.e.
+//   (f0.n0)(a n . dx) - a f . dx + a n . (dx x (n0 x f0)).
+// Tgt.sub holds f0.n0 and Tgt.g0 holds n0 x f0.  Every pair evaluates it in
+// its source subtile's frame as one FMA chain (p_chain, below): each product
+// reads at most one source value (the wave holds the record in scalar
+// registers and a VALU instruction reads at most one).
 
 // Per-target data of the pressure's plain pairs in the frame of one source
 // subtile (box centre c): p' = p - c, m2 = -2 p', pp = |p'|^2 and
@@ -610,16 +602,15 @@ __device__ __forceinline__ void make_tgtp(const Tgt& t, const double* box, TgtP&
   v.pp = v.px * v.px + v.py * v.py + v.pz * v.pz;
 }
 
-// Plain kernel H2 = 1/r^3 (Lapl_DL_7ord beyond 8 delta), everything in the
-// subtile's frame (d = p' - x').  r^2 = (|p'|^2 + |x'|^2) - 2 p'.x' is one
-// FMA chain on the record's xx; the numerator (f0.n0)(n.d) - f.d + n.(d x c),
-// c = n0 x f0, is n.q - f.p' - c.m + (be - (f0.n0) al), one FMA chain on the
-// record's precomputed al, be, m.  The terms of both are of size |p'| + |x'|,
-// where |x'| is within the subtile's box and |p'| - |x'| <= r with r > 8 delta
-// for every plain pair, so their rounding stays within a few ulps of r^2 and
-// of the numerator.
-__device__ __forceinline__ void p_plain_r2(const SrcRecP& s, const Tgt& t, const TgtP& v, double r2, double k3,
-                                           double u[3]) {
+// The numerator in the subtile's frame (d = p' - x'): (f0.n0)(n.d) - f.d +
+// n.(d x c), c = n0 x f0, is n.q - f.p' - c.m + (be - (f0.n0) al), one FMA
+// chain on the record's precomputed al, be, m.  For the plain kernel H2 = 1/r^3
+// (Lapl_DL_7ord beyond 8 delta) r^2 = (|p'|^2 + |x'|^2) - 2 p'.x' is a second
+// chain on the record's xx.  The terms of both are of size |p'| + |x'|, where
+// |x'| is within the subtile's box and |p'| - |x'| <= r, so their rounding
+// stays within a few ulps times (|p'| + |x'|) / r: a few ulps for the plain
+// pairs (r > 8 delta), a few 1e-15 relative for the closest regularized ones.
+__device__ __forceinline__ double p_chain(const SrcRecP& s, const Tgt& t, const TgtP& v) {
   double a = v.qx * s.nx;
   a = __builtin_fma(v.qy, s.ny, a);
   a = __builtin_fma(v.qz, s.nz, a);
@@ -630,8 +621,12 @@ __device__ __forceinline__ void p_plain_r2(const SrcRecP& s, const Tgt& t, const
   a = __builtin_fma(-t.g0y, s.m1, a);
   a = __builtin_fma(-t.g0z, s.m2, a);
   a = __builtin_fma(-t.sub, s.al, a);
-  a = a + s.be;
-  u[0] += a * rsqrt3_64(r2, k3);
+  return a + s.be;
+}
+
+__device__ __forceinline__ void p_plain_r2(const SrcRecP& s, const Tgt& t, const TgtP& v, double r2, double k3,
+                                           double u[3]) {
+  u[0] += p_chain(s, t, v) * rsqrt3_64(r2, k3);
 }
 
 __device__ __forceinline__ void p_plain(const SrcRecP& s, const Tgt& t, const TgtP& v, double k3, double u[3]) {
@@ -643,16 +638,19 @@ __device__ __forceinline__ void p_plain(const SrcRecP& s, const Tgt& t, const Tg
 }
 
 // Full branch structure of Lapl_DL_7ord (extend_to_grid/utilities.cpp:190-211)
-// at a pair of a mixed subtile, d = p' - x' and r^2 from the vote: self limit,
-// plain, or (erf + P2(rho^2) rho e^{-rho^2}) / r^3 with the Stokes_SL_7ord s2
-// polynomial (the same coefficients q[3..6]).
-__device__ __forceinline__ void p_full(const SrcRecP& s, const Tgt& t, double dx, double dy, double dz, double r2,
-                                       double inv_del, double inv_del2, double self2,
-                                       const double* __restrict__ tab, double u[3]) {
+// at a pair of a mixed subtile, r^2 from the vote: plain, or
+// (erf + P2(rho^2) rho e^{-rho^2}) / r^3 with the Stokes_SL_7ord s2 polynomial
+// (the same coefficients q[3..6]), times the numerator as the plain pair's FMA
+// chain in the subtile frame.  Its terms are of size |p'| + |x'| <= r + 2 x
+// (the box radius), so a near pair's numerator is within a few ulps times
+// (|p'| + |x'|) / r of d.w (a few 1e-15 at the closest band pairs).
+__device__ __forceinline__ void p_full(const SrcRecP& s, const Tgt& t, const TgtP& v, double r2, double inv_del,
+                                       double inv_del2, const double* __restrict__ tab, double u[3]) {
+  // d = 0: the numerator, linear in d, vanishes (the reference multiplies it
+  // by the self limit of H2)
+  if (t.nearp && r2 < 1e-28) return;
   double H2;
-  if (t.nearp && r2 < 1e-28) {
-    H2 = self2;
-  } else {
+  {
     const double ri = rsqrt64(r2);
     const double sq = r2 * inv_del2;   // rho^2
     if (!t.nearp || sq > 64.0) {
@@ -664,7 +662,7 @@ __device__ __forceinline__ void p_full(const SrcRecP& s, const Tgt& t, double dx
       H2 = (A + p2 * E) * (inv_del * (ri * ri));
     }
   }
-  u[0] += p_numer(s, t, dx, dy, dz) * H2;
+  u[0] += p_chain(s, t, v) * H2;
 }
 
 // Target data of sum kernel KIND.  NOSUB (double layer): the pair terms run on
@@ -807,10 +805,8 @@ __device__ __forceinline__ void range_sum(const DirectArgs& A, int j0, int j1, i
 __device__ __forceinline__ void range_sum_p(const DirectArgs& A, int j0, int j1, int sub0, const double* wb,
                                             bool all_far, const Tgt& t, const double* __restrict__ tab,
                                             double u[3]) {
-  const double rp = 1.7724538509055160273;
   const double inv_del = 1.0 / A.DEL;
   const double inv_del2 = inv_del * inv_del;
-  const double self2 = 256.0 / 15.0 / rp / (A.DEL * A.DEL * A.DEL);
   const SrcRecP* __restrict__ R = A.recp;
   const unsigned long long nearm = wave_mask(t.nearp);   // lanes whose target is near its surface
   const double k3 = vreg(1.875);   // rsqrt3_64's e^2 coefficient, in vector registers for the loops
@@ -827,7 +823,7 @@ __device__ __forceinline__ void range_sum_p(const DirectArgs& A, int j0, int j1,
         const SrcRecP s = R[j];
         const double dx = v.px - s.x, dy = v.py - s.y, dz = v.pz - s.z;
         const double r2 = dx * dx + dy * dy + dz * dz;
-        if (wave_mask(!(r2 > A.near_r2)) & nearm) p_full(s, t, dx, dy, dz, r2, inv_del, inv_del2, self2, tab, u);
+        if (wave_mask(!(r2 > A.near_r2)) & nearm) p_full(s, t, v, r2, inv_del, inv_del2, tab, u);
         else p_plain_r2(s, t, v, r2, k3, u);
       }
     }
