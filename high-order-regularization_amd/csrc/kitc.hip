@@ -15,8 +15,8 @@
 //                  targets (same pair code and LDS tiling as the direct kernel);
 //   3. k_far:      plain Stokeslet / stresslet from the (P+1)^3 proxies with
 //                  target-side subtraction (Call_Treecode_SL/DL);
-//   4. k_tree_finalize: one wavefront per target adds its slots (a fixed-order
-//                  reduction), scales, subtracts and adds the jump term.
+//   4. k_tree_finalize: 16 lanes per target add its slots (a fixed-order
+//                  reduction), scale, subtract and add the jump term.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -768,29 +768,31 @@ __global__ void k_csub_store(int nt, int ntp, int l, const int* tperm, const dou
   for (int k = 0; k < 3; k++) csub[(size_t)(3 * k + l) * ntp + i] = col[3 * (size_t)o + k];
 }
 
-// Per-target reduction of the slots (one wavefront per target): lane l adds
-// the target's slots a + l, a + l + 64, ... of each component (coalesced
-// reads of the target's contiguous slot range), a butterfly adds the 64
-// partial sums (every lane ends with the same total, in the same order on
-// every run), and lane k < 3 finishes component k: v = scale*sum, minus the
-// subtraction coefficients' C g0 (NOSUB), plus the jump term xi*g0, combined
-// into out as the direct finalize does.
+// Per-target reduction of the slots (16 lanes per target, four targets per
+// wavefront): lane l of a target's group adds its slots a + l, a + l + 16, ...
+// of each component (coalesced reads of the target's contiguous slot range),
+// a butterfly adds the 16 partial sums (every lane of the group ends with the
+// same total, in the same order on every run), and lane k < 3 finishes
+// component k: v = scale*sum, minus the subtraction coefficients' C g0
+// (NOSUB), plus the jump term xi*g0, combined into out as the direct finalize
+// does.
+constexpr int kFinLanes = 16;   // lanes per target
 __global__ __launch_bounds__(kBlock) void k_tree_finalize(int nt, int ntp, int nslot, const int* tperm,
                                                           const int* slot_start, const double* slots, double scale,
                                                           const double* xi, const double* g0, const double* csub,
                                                           double* out, const double* base, double coef,
                                                           int accumulate) {
-  const int lane = threadIdx.x & (kWave - 1);
-  const int i = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
-  if (i >= nt) return;   // the whole wave
+  const int lane = threadIdx.x & (kFinLanes - 1);
+  const int i = blockIdx.x * (kBlock / kFinLanes) + threadIdx.x / kFinLanes;
+  if (i >= nt) return;   // the target's whole lane group
   const int a = slot_start[i], b = slot_start[i + 1];
   double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-  for (int q = a + lane; q < b; q += kWave) {
+  for (int q = a + lane; q < b; q += kFinLanes) {
     s0 += slots[q];
     s1 += slots[(size_t)nslot + q];
     s2 += slots[2 * (size_t)nslot + q];
   }
-  for (int m = kWave / 2; m > 0; m >>= 1) {
+  for (int m = kFinLanes / 2; m > 0; m >>= 1) {
     s0 += __shfl_xor(s0, m);
     s1 += __shfl_xor(s1, m);
     s2 += __shfl_xor(s2, m);
@@ -1244,7 +1246,7 @@ static int tree_eval(hor_plan* P, const double* dens_dev, const double* d0_dev, 
   HOR_HIP_TRY(hipEventRecord(TP.ev[3], s));
   TP.evaluated = true;
   // 4. per-target reduction
-  hipLaunchKernelGGL(k_tree_finalize, dim3((P->nt + kBlock / kWave - 1) / (kBlock / kWave)), dim3(kBlock), 0, s,
+  hipLaunchKernelGGL(k_tree_finalize, dim3((P->nt + kBlock / kFinLanes - 1) / (kBlock / kFinLanes)), dim3(kBlock), 0, s,
                      P->nt, P->ntp,
                      TP.nslot, P->tperm.p, TP.slot_start.p, TP.slots.p,
                      dl ? -6.0 / (8.0 * kPi) : 1.0 / (8.0 * kPi),   // KITC.cpp:1395, 1612
